@@ -1,0 +1,372 @@
+"""Pins the FP64 oracle before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the oracle (dropped term,
+wrong sign or index, transposed operand, broken exponent scaling) fails it:
+literal vectors from the reference's own tests, closed forms, independent
+library routines (scipy, numpy.fft, sympy), brute force on tiny inputs, and
+invariants (orthonormality, adjointness, rotation).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import pathlib
+
+import numpy as np
+import pytest
+from scipy.special import roots_legendre, sph_harm_y
+
+from oracle import oracle as O
+
+GOLD = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+
+
+def _hp(lmax, l, m):
+    return m * (2 * lmax + 1 - m) // 2 + l
+
+
+def _brute_synth(alm, lmax, theta, phi):
+    out = np.zeros_like(theta)
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            a = alm[_hp(lmax, l, m)]
+            y = sph_harm_y(l, m, theta, phi)
+            out += (a.real * y).real if m == 0 else 2 * (a * y).real
+    return out
+
+
+# --------------------------------------------------------------------------
+# reference-owned arithmetic against the reference tests' literal vectors
+
+
+@pytest.mark.parametrize("nd", [False, True])
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_iternorm_against_pinv_cholesky(k, nd):
+    # tests/core/test_fields.py:31-104
+    cov = np.asarray(GOLD["iternorm_cov_nd" if nd else "iternorm_cov_0d"])
+    rows = (cov[..., i, i::-1][..., : min(i, k) + 1] for i in range(cov.shape[-1]))
+    for n, w in enumerate(O.iternorm(rows)):
+        Sn = cov[..., :n, :n].copy()
+        cn = cov[..., :n, n].copy()
+        vn = cov[..., n, n]
+        if n > k:
+            band = (np.abs(np.arange(n)[:, None] - np.arange(n)) <= k).astype(float)
+            Sn *= band
+            cn *= (np.arange(n) >= n - k).astype(float)
+        Sinv = np.linalg.pinv(Sn)
+        An = np.swapaxes(np.linalg.cholesky(Sinv + 1e-100 * np.eye(n)), -1, -2)
+        an = (An @ cn[..., None])[..., 0]
+        sn = np.sqrt(vn - np.einsum("...i,...i", an, an))
+        assert w.shape[-1] <= min(n, k) + 1
+        a, s = w[..., :-1], w[..., -1]
+        np.testing.assert_allclose(np.einsum("...i,...i", a, a),
+                                   np.einsum("...i,...i", an, an), rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(s, sn, rtol=1e-12)
+
+
+def test_iternorm_errors():
+    # tests/core/test_fields.py:107-119
+    with pytest.raises(ValueError, match="empty covariance"):
+        list(O.iternorm([np.ones(0)]))
+    with pytest.raises(ValueError, match="shape mismatch"):
+        list(O.iternorm([np.ones(1), np.ones((5, 2))]))
+    with pytest.raises(ValueError, match="not positive definite"):
+        list(O.iternorm([np.asarray([1.0]), np.asarray([0.1, 1.0])]))
+
+
+def test_cls2cov_vectors():
+    # tests/core/test_fields.py:166-252
+    g = GOLD["cls2cov_single"]
+    cls = [None if c is None else np.asarray(c) for c in g["cls"]]
+    cov = next(O.cls2cov(cls, g["nl"], g["nf"], g["nc"]))
+    assert cov.shape == (3, 3)
+    np.testing.assert_array_equal(cov[:, 0], g["col0"])
+    np.testing.assert_array_equal(cov[:, 1:], 0.0)
+
+    g = GOLD["cls2cov_negative"]
+    with pytest.raises(ValueError, match="negative values in cl"):
+        next(O.cls2cov([np.asarray(c) for c in g["cls"]], g["nl"], g["nf"], g["nc"]))
+
+    g = GOLD["cls2cov_multi"]
+    gen = O.cls2cov([np.asarray(c) for c in g["cls"]], g["nl"], g["nf"], g["nc"])
+    c1 = next(gen)
+    c1_copy = c1.copy()
+    c2 = next(gen)
+    c3 = next(gen)
+    # same buffer reused for every yield
+    assert c1 is c2 is c3
+    np.testing.assert_array_equal(c3[:, 0], g["final_col0"])
+    np.testing.assert_array_equal(c1_copy[:, 0], g["first_col0"])
+
+
+def test_multalm_vectors():
+    # tests/core/test_harmonics.py:15-52
+    for case in GOLD["multalm"]:
+        out = O.multalm(np.asarray(case["alm"]), np.asarray(case["bl"]))
+        np.testing.assert_array_equal(out, case["out"])
+
+
+def test_glass_to_healpix_alm():
+    g = GOLD["glass_to_healpix_alm"]
+    out = O.glass_to_healpix_alm(np.asarray(g["in"], dtype=np.complex128))
+    np.testing.assert_array_equal(out, np.asarray(g["out"], dtype=np.complex128))
+    # closed form: GLASS l(l+1)/2 + m lands at healpy m(2 lmax + 1 - m)/2 + l
+    lmax = 9
+    idx = np.asarray([l * (l + 1) // 2 + m for l in range(lmax + 1) for m in range(l + 1)])
+    x = np.arange(idx.size).astype(np.complex128)
+    y = O.glass_to_healpix_alm(x)
+    for l in range(lmax + 1):
+        for m in range(l + 1):
+            assert y[_hp(lmax, l, m)] == l * (l + 1) // 2 + m
+
+
+def test_rng_stream_pinned():
+    z = np.random.default_rng(42).standard_normal((6, 2))
+    np.testing.assert_array_equal(z, np.asarray(GOLD["pcg64_seed42_normals_6x2"]))
+
+
+def test_transform_closed_forms():
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(10)
+    for lam in (1.0, 0.37):
+        var = 0.6
+        np.testing.assert_array_equal(O.lognormal(x, var, lam), lam * np.expm1(x - var / 2))
+        a = math.sqrt(1 - var)
+        np.testing.assert_array_equal(O.squared_normal(x, a, lam), lam * ((x - a) ** 2 - 1))
+
+
+def test_getcl_and_nfields():
+    cls = [np.asarray([i, j], dtype=float) for i in range(6) for j in range(i, -1, -1)]
+    for i in range(6):
+        for j in range(6):
+            np.testing.assert_array_equal(O.getcl(cls, i, j), sorted([i, j], reverse=True))
+    assert O.nfields_from_nspectra(10) == 4
+    with pytest.raises(ValueError, match="invalid number of spectra"):
+        O.nfields_from_nspectra(7)
+
+
+# --------------------------------------------------------------------------
+# geometry and transforms
+
+
+def test_ring_geometry():
+    for nside in (1, 2, 4, 16):
+        npix = 12 * nside * nside
+        total = 0
+        prev_z = 2.0
+        for r in range(1, 4 * nside):
+            s, n, z, st, p0 = O.ring_info(nside, r)
+            assert s == total
+            total += n
+            assert z < prev_z
+            prev_z = z
+            assert abs(st * st + z * z - 1) < 1e-15
+        assert total == npix
+    # nside 1: 4 pixels per ring, first pixel of ring 2 at phi = 0, ring 1 at pi/4
+    assert O.ring_info(1, 1)[4] == pytest.approx(math.pi / 4)
+    assert O.ring_info(1, 2)[4] == 0.0
+    assert O.ring_info(1, 2)[2] == 0.0
+    # equal-area: mean z over pixels is 0 and pixels are uniform in z per cap
+    th, _ = O.pixel_angles(8)
+    assert abs(np.mean(np.cos(th))) < 1e-14
+
+
+def test_closed_forms_low_l():
+    nside, lmax = 4, 2
+    th, ph = O.pixel_angles(nside)
+    # Y_00
+    a = np.zeros(O.nalm(lmax), complex)
+    a[_hp(lmax, 0, 0)] = 1.0
+    np.testing.assert_allclose(O.alm2map(a, nside), 1 / math.sqrt(4 * math.pi), rtol=1e-14)
+    # Y_10
+    a[:] = 0
+    a[_hp(lmax, 1, 0)] = 1.0
+    np.testing.assert_allclose(O.alm2map(a, nside), math.sqrt(3 / (4 * math.pi)) * np.cos(th),
+                               atol=1e-15)
+    # a_11 = 1: 2 Re(Y_11) = -2 sqrt(3/8pi) sin th cos phi (Condon-Shortley)
+    a[:] = 0
+    a[_hp(lmax, 1, 1)] = 1.0
+    np.testing.assert_allclose(O.alm2map(a, nside),
+                               -2 * math.sqrt(3 / (8 * math.pi)) * np.sin(th) * np.cos(ph),
+                               atol=1e-15)
+
+
+@pytest.mark.parametrize("nside,lmax", [(1, 3), (2, 5), (4, 11), (8, 23)])
+def test_alm2map_brute_force(nside, lmax):
+    rng = np.random.default_rng(nside)
+    alm = rng.standard_normal(O.nalm(lmax)) + 1j * rng.standard_normal(O.nalm(lmax))
+    th, ph = O.pixel_angles(nside)
+    ref = _brute_synth(alm, lmax, th, ph)
+    got = O.alm2map(alm, nside, nthreads=2)
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_ring_dft_against_numpy():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 3, 4, 7, 12, 20, 28, 44, 64, 97, 100, 8156):
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        np.testing.assert_allclose(O.dft(x, -1), np.fft.fft(x), rtol=0, atol=1e-12 * n)
+        np.testing.assert_allclose(O.dft(x, +1), np.fft.ifft(x) * n, rtol=0, atol=1e-12 * n)
+
+
+def test_legendre_orthonormal_high_l():
+    """Gauss-Legendre quadrature is exact for lambda_lm lambda_l'm when l + l' < 2N,
+    so the Gram matrix must be the identity / 2 pi.  Covers l up to 2500 with
+    seeds that underflow FP64 by hundreds of orders of magnitude near the poles."""
+    lmax = 2500
+    N = lmax + 1
+    x, w = roots_legendre(N)
+    th = np.arccos(x)
+    for m in (0, 1, 700, 2400):
+        lam = np.stack([O.legendre(m, lmax, t) for t in th])  # (N, lmax-m+1)
+        gram = 2 * np.pi * (lam * w[:, None]).T @ lam
+        err = np.abs(gram - np.eye(gram.shape[0]))
+        # the median entry is at round-off (~1e-14); the worst entries at m <= 1
+        # are limited by the accuracy of the polar quadrature nodes themselves
+        assert np.median(err) < 1e-13, (m, np.median(err))
+        assert err.max() < 1e-9, (m, err.max())
+
+
+def test_legendre_scaling_against_arbitrary_exponent():
+    """mpmath floats have an unbounded exponent: run the recurrence there without
+    any rescaling and compare, at seeds far below the FP64 range."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    # seeds sin^m(theta) of 1e-430, 1e-700, 1e-1050, 1e-346; the functions turn
+    # on (l ~ m / sin theta) before lmax so the compared tail is O(1)
+    for m, lmax, theta in [(4000, 6000, 0.9), (1000, 6000, 0.2), (1500, 8000, 0.2),
+                           (300, 5000, 0.07)]:
+        st, x = mp.sin(mp.mpf(theta)), mp.cos(mp.mpf(theta))
+        v = 1 / mp.sqrt(4 * mp.pi)
+        for k in range(1, m + 1):
+            v *= -mp.sqrt(mp.mpf(2 * k + 1) / (2 * k)) * st
+        p0, p1 = mp.mpf(0), v
+        vals = [p1]
+        for l in range(m + 1, lmax + 1):
+            a = mp.sqrt(mp.mpf(4 * l * l - 1) / (l * l - m * m))
+            b = mp.sqrt(mp.mpf((l - 1) ** 2 - m * m) / (4 * (l - 1) ** 2 - 1))
+            p0, p1 = p1, a * (x * p1 - b * p0)
+            vals.append(p1)
+        got = O.legendre(m, lmax, theta)
+        ref = np.asarray([float(v) for v in vals])
+        scale = np.max(np.abs(ref))
+        assert scale > 1e-3
+        assert np.max(np.abs(got - ref)) < 1e-12 * scale
+
+
+def test_phi_rotation_invariant():
+    """Rotating alm by exp(-i m alpha) with alpha = 2 pi / n on the equator
+    shifts that ring by one pixel: f_rot(phi) = f(phi - alpha)."""
+    nside, lmax = 8, 20
+    rng = np.random.default_rng(5)
+    alm = rng.standard_normal(O.nalm(lmax)) + 1j * rng.standard_normal(O.nalm(lmax))
+    for l in range(lmax + 1):
+        alm[_hp(lmax, l, 0)] = alm[_hp(lmax, l, 0)].real
+    alpha = 2 * np.pi / (4 * nside)
+    rot = alm.copy()
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            rot[_hp(lmax, l, m)] *= np.exp(-1j * m * alpha)
+    a = O.alm2map(alm, nside)
+    b = O.alm2map(rot, nside)
+    s, n, *_ = O.ring_info(nside, 2 * nside)
+    np.testing.assert_allclose(b[s:s + n], np.roll(a[s:s + n], 1), atol=1e-12)
+
+
+def test_map2alm_adjoint_and_round_trip():
+    nside, lmax = 8, 12
+    npix = 12 * nside * nside
+    rng = np.random.default_rng(11)
+    alm = rng.standard_normal(O.nalm(lmax)) + 1j * rng.standard_normal(O.nalm(lmax))
+    for l in range(lmax + 1):
+        alm[_hp(lmax, l, 0)] = alm[_hp(lmax, l, 0)].real
+    f = rng.standard_normal(npix)
+    Sa = O.alm2map(alm, nside)
+    Af = O.map2alm(f, lmax, niter=0)
+    # <S a, f> = (npix/4pi) Re sum_m (2 - delta_m0) conj(a) (A f)
+    lhs = np.dot(Sa, f)
+    w = np.asarray([1.0 if m == 0 else 2.0 for m in range(lmax + 1)
+                    for _ in range(m, lmax + 1)])
+    rhs = npix / (4 * np.pi) * np.sum(w * (np.conj(alm) * Af).real)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+    # band-limited round trip converges with Jacobi iterations
+    e0, e3, e10 = (np.max(np.abs(O.map2alm(Sa, lmax, niter=k) - alm)) for k in (0, 3, 10))
+    assert e3 < e0 / 100
+    assert e10 < 1e-10
+
+
+def test_cltovar_against_map_variance():
+    nside, lmax = 16, 30
+    cl = 1.0 / (np.arange(lmax + 1) + 1.0) ** 2
+    rng = np.random.default_rng(2)
+    vals = []
+    for _ in range(40):
+        alm = np.zeros(O.nalm(lmax), complex)
+        for m in range(lmax + 1):
+            for l in range(m, lmax + 1):
+                if m == 0:
+                    alm[_hp(lmax, l, 0)] = rng.standard_normal() * math.sqrt(cl[l])
+                else:
+                    alm[_hp(lmax, l, m)] = (rng.standard_normal() + 1j * rng.standard_normal()) \
+                        * math.sqrt(cl[l] / 2)
+        vals.append(np.mean(O.alm2map(alm, nside) ** 2))
+    assert abs(np.mean(vals) / O.cltovar(cl) - 1) < 0.1
+
+
+def test_spin_against_eth_operator():
+    """sY_lm from the spin-raising operator applied symbolically to Y_lm, and
+    the oracle's [Q, U] synthesis against -sum_{l, |m| <= l} (E + iB) sY_lm
+    with E_{l,-m} = (-1)^m conj(E_lm) evaluated by brute force."""
+    import sympy as sp
+
+    th, ph = sp.symbols("theta phi", real=True)
+
+    def eth(f, s):
+        g = sp.sin(th) ** (-s) * f
+        return -sp.sin(th) ** s * (sp.diff(g, th) + sp.I / sp.sin(th) * sp.diff(g, ph))
+
+    def sY(l, m, s):
+        f = sp.Ynm(l, m, th, ph).expand(func=True)
+        for k in range(s):
+            f = eth(f, k)
+        return sp.lambdify((th, ph), f * sp.sqrt(sp.factorial(l - s) / sp.factorial(l + s)),
+                           "numpy")
+
+    lmax, nside = 4, 2
+    for s in (1, 2):
+        rng = np.random.default_rng(s)
+        E = rng.standard_normal(O.nalm(lmax)) + 1j * rng.standard_normal(O.nalm(lmax))
+        B = rng.standard_normal(O.nalm(lmax)) + 1j * rng.standard_normal(O.nalm(lmax))
+        for l in range(lmax + 1):
+            E[_hp(lmax, l, 0)] = E[_hp(lmax, l, 0)].real
+            B[_hp(lmax, l, 0)] = B[_hp(lmax, l, 0)].real
+        theta, phi = O.pixel_angles(nside)
+        P = np.zeros(theta.size, complex)
+        for l in range(s, lmax + 1):
+            for m in range(-l, l + 1):
+                am = abs(m)
+                e, b = E[_hp(lmax, l, am)], B[_hp(lmax, l, am)]
+                if m < 0:
+                    e, b = (-1) ** am * np.conj(e), (-1) ** am * np.conj(b)
+                P += -(e + 1j * b) * sY(l, m, s)(theta, phi)
+        q, u = O.alm2map_spin([E, B], nside, s, lmax)
+        np.testing.assert_allclose(q, P.real, atol=1e-12)
+        np.testing.assert_allclose(u, P.imag, atol=1e-12)
+
+
+def test_generate_invariants():
+    """The reference's own invariants for generate (test_fields.py:362-438)."""
+    gls = [np.asarray([1.0, 0.5, 0.1])]
+    a = list(O.generate_grf_alm(gls, rng=np.random.default_rng(42)))
+    b = list(O.generate_grf_alm(gls, ncorr=1, rng=np.random.default_rng(42)))
+    np.testing.assert_array_equal(a[0], b[0])
+    with pytest.raises(ValueError, match="all gls are empty"):
+        list(O.generate_grf_alm([np.asarray([])]))
+    gls = [np.ones(10), np.ones(10), np.ones(10)]
+    fields = [lambda x, var: x, lambda x, var: x**2]
+    res = list(O.generate(fields, gls, nside=4, rng=np.random.default_rng(1)))
+    np.testing.assert_allclose(res[1], res[0] ** 2, atol=1e-5)
+    with pytest.raises(ValueError, match="mismatch between number of fields and gls"):
+        list(O.generate(fields, [np.ones(10), np.ones(10)], nside=4))
