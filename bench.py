@@ -1,0 +1,199 @@
+"""Throughput of the lognormal-shell generation path on one MI355X.
+
+Workload (the headline configuration): S correlated lognormal shells at
+nside = 2048, lmax = 4096, ncorr = 3 (neighbour correlation 0.6 / 0.3 / 0.1,
+auto-spectra 1e-4 (l+1)^-1.2 exp(-(l/1500)^2) with amplitude rising x1 -> x4
+across shells), normals from the device Philox generator.  One step = one
+batch of S shells through the whole path, inputs resident in HBM:
+
+    iternorm -> correlated alm (+ healpy reorder, m = 0 fix)
+    -> Legendre stage on FP32 MFMA (FP64 recurrence) -> ring FFT + lognormal.
+
+value = maps/s over the timed steps.  Also reported: the plain FP64 GPU path's
+maps/s on the same configuration, the FP32 maps' error against the FP64 path
+(same normals), the roofline of the Legendre kernel (achieved dense FLOP/s
+from HIP events on the launch stream), per-step p50/p99 latency, and the FP64
+CPU oracle timed on a bounded sample on the host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+MFMA_FP32_PEAK = 157.3e12  # MI355X dense FP32 matrix FLOP/s
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nside", type=int, default=2048)
+    ap.add_argument("--lmax", type=int, default=None)
+    ap.add_argument("--shells", type=int, default=64)
+    ap.add_argument("--fp64-maps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def read_traffic(nside: int, lmax: int, shells: int):
+    """HBM bytes per launch of the Legendre kernel from the committed rocprof
+    PMC summary (a line 'traffic_bytes_per_launch: <n>' for this workload)."""
+    p = ROOT / "profiles" / f"legendre_mfma_nside{nside}_lmax{lmax}_S{shells}_pmc.md"
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(nside: int, lmax: int, gls, step: int):
+    """The FP64 oracle on the host: one shell of the workload, full pipeline
+    (cls2cov, iternorm, numpy PCG64 normals, multalm sum, healpy reorder,
+    synthesis, lognormal), synthesis on every `step`-th ring, scaled to a map."""
+    from oracle import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    alm = next(O.generate_grf_alm(gls[:1], ncorr=0, rng=np.random.default_rng(0)))
+    t_alm = time.perf_counter() - t0
+    nr = 4 * nside - 1
+    t0 = time.perf_counter()
+    x = O.alm2map_rings(alm, nside, 1, step, nthreads=cores)
+    t_synth = (time.perf_counter() - t0) * nr / len(range(1, nr + 1, step))
+    t0 = time.perf_counter()
+    O.lognormal(x, O.cltovar(gls[0]))
+    t_ln = time.perf_counter() - t0
+    total = t_alm + t_synth + t_ln
+    return {
+        "value": 1.0 / total, "unit": "maps/s", "cores": cores, "kind": "port",
+        "sample": (f"1 lognormal shell at nside {nside}, lmax {lmax} through the FP64 C/numpy "
+                   f"oracle on {cores} host threads; synthesis timed on every {step}th ring "
+                   f"({len(range(1, nr + 1, step))} of {nr}) and scaled to the full map; "
+                   f"alm {t_alm:.2f} s, synthesis {t_synth:.1f} s (scaled), lognormal "
+                   f"{t_ln:.2f} s"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(shells and requests are independent, so N GPUs run N independent batches).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import glass_amd
+    from glass_amd.batch import LognormalShells, shell_spectra
+
+    nside = args.nside
+    lmax = args.lmax if args.lmax is not None else 2 * nside
+    S = args.shells
+    torch.cuda.set_device(0)
+    gls = shell_spectra(S, lmax, amp=lambda i: 1.0 + 3.0 * i / max(S - 1, 1),
+                        rho=(0.6, 0.3, 0.1), scale=1500.0)
+    print(f"[bench] building batch: nside={nside} lmax={lmax} shells={S}", file=sys.stderr,
+          flush=True)
+    eng = LognormalShells(gls, nside, ncorr=3, shift=1.0, seed=1234, lmax=lmax)
+    for _ in range(args.warmup):
+        eng.run()
+    torch.cuda.synchronize()
+    eng.check()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        e = ev[k]
+        e[0].record()
+        eng.stage_fields()
+        e[1].record()
+        eng.stage_legendre()
+        e[2].record()
+        eng.stage_rings()
+        e[3].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.check()
+    wall = t1 - t0
+    step_ms = [ev[k][0].elapsed_time(ev[k][3]) for k in range(args.steps)]
+    fields_ms = float(np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)]))
+    leg_ms = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)]))
+    ring_ms = float(np.mean([ev[k][2].elapsed_time(ev[k][3]) for k in range(args.steps)]))
+    value = args.steps * S / wall
+    print(f"[bench] timed {args.steps} steps: {wall:.3f} s", file=sys.stderr, flush=True)
+
+    # accuracy of the FP32 maps against the FP64 path on identical normals
+    errs = []
+    for s in sorted({0, S // 2, S - 1}):
+        ref = eng.fp64_reference_map(s)
+        got = eng.maps[s].double()
+        rms = torch.sqrt(torch.mean(ref * ref))
+        errs.append({"shell": s, "max_rel": float(torch.max(torch.abs(got - ref)) / rms),
+                     "rms_rel": float(torch.sqrt(torch.mean((got - ref) ** 2)) / rms)})
+
+    # plain FP64 GPU path: one map per launch
+    fp64 = None
+    if args.fp64_maps > 0:
+        eng.fp64_reference_map(0)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for i in range(args.fp64_maps):
+            eng.fp64_reference_map(i % S)
+        torch.cuda.synchronize()
+        fp64 = args.fp64_maps / (time.perf_counter() - ta)
+
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    flop = 4.0 * nalm * (2 * nside) * S  # dense Legendre contraction, ring pairs
+    achieved = flop / (leg_ms * 1e-3)
+    traffic = read_traffic(nside, lmax, S)
+    out = {
+        "metric": "lognormal maps/s at nside 2048 (FP32-MFMA vs FP64 plain, same GPU); p99 latency",
+        "value": value,
+        "unit": "maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "batched lognormal shells", "nside": nside, "lmax": lmax,
+                   "shells_per_batch": S, "ncorr": 3, "rng": "device Philox4x32-10",
+                   "stages": "iternorm + alm + FP32-MFMA Legendre + ring FFT + lognormal"},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
+                     "kernel": "legendre_synth_mfma", "flop_per_launch": flop,
+                     "avg_launch_ms": leg_ms},
+        "stages_ms": {"fields": fields_ms, "legendre_mfma": leg_ms, "ring_fft_lognormal": ring_ms},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                       "p99": float(np.percentile(step_ms, 99)), "unit": "per batch"},
+        "fp64_plain_maps_per_s": fp64,
+        "speedup_vs_fp64_plain": (value / fp64) if fp64 else None,
+        "accuracy_vs_fp64": errs,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        step = 32 if nside >= 1024 else 1
+        out["cpu_baseline"] = cpu_baseline(nside, lmax, gls, step)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,35 @@
+"""glass_amd: MI355X-native field-generation hot path of GLASS.
+
+Drop-in counterparts of the reference's generation path (same names, argument
+meaning and error messages):
+
+    glass_amd.generate / generate_lognormal / generate_gaussian / iternorm /
+    cls2cov / getcl / nfields_from_nspectra          (glass/fields.py)
+    glass_amd.harmonics.multalm                      (glass/harmonics.py)
+    glass_amd.healpix.alm2map / map2alm / alm2map_spin / almxfl / ...
+                                                     (glass/healpix.py)
+    glass_amd.grf.Normal / Lognormal / SquaredNormal (glass/grf)
+    glass_amd.from_convergence / shear_from_convergence (glass/lensing.py)
+
+All numerics run in hand-written HIP kernels for gfx950 behind the C ABI in
+include/glass_amd.h; there is no CPU fallback.
+"""
+
+from . import grf, harmonics, healpix, rng
+from .fields import (
+    cls2cov,
+    generate,
+    generate_gaussian,
+    generate_lognormal,
+    getcl,
+    iternorm,
+    nfields_from_nspectra,
+)
+from .lensing import from_convergence, shear_from_convergence
+from .rng import DeviceRNG
+
+__all__ = [
+    "DeviceRNG", "cls2cov", "from_convergence", "generate", "generate_gaussian",
+    "generate_lognormal", "getcl", "grf", "harmonics", "healpix", "iternorm",
+    "nfields_from_nspectra", "rng", "shear_from_convergence",
+]

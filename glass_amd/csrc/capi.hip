@@ -1,0 +1,174 @@
+// extern "C" entry points of the spherical-harmonic transforms: argument
+// checks, workspace carving and the kernel sequence of each transform.
+#include "common.h"
+#include "device.h"
+#include "kernels.h"
+
+namespace ga {
+namespace {
+
+template <typename T>
+__global__ void residual_kernel(int64_t n, const T *__restrict__ a, T *__restrict__ b) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i] - b[i];
+}
+
+size_t ring_bytes(const ga_plan *p, int nbatch, int rb) {
+  return size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nbatch) * 2 * size_t(rb);
+}
+size_t partial_bytes(const ga_plan *p, int nbatch, int rb) {
+  return size_t(anal_partial_slots(p)) * size_t(p->nalm) * size_t(nbatch) * 2 * size_t(rb);
+}
+
+}  // namespace
+}  // namespace ga
+
+using namespace ga;
+
+#define GA_PLAN_OK(p)                                                                      \
+  do {                                                                                     \
+    clear_error();                                                                         \
+    GA_REQUIRE((p) != nullptr, GA_ERR_INVALID, "plan is NULL");                            \
+  } while (0)
+
+extern "C" int ga_map2alm_workspace_size(const ga_plan *p, int32_t nbatch, int32_t rb,
+                                         size_t *bytes) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(bytes && nbatch >= 1 && (rb == 4 || rb == 8), GA_ERR_INVALID, "invalid argument");
+  *bytes = ring_bytes(p, nbatch, rb) + partial_bytes(p, nbatch, rb);
+  return GA_OK;
+}
+
+extern "C" int ga_plan_prepare_spin(ga_plan *p, int32_t spin) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(spin == 1 || spin == 2, GA_ERR_UNSUPPORTED, "spin %d not supported (1 or 2)", spin);
+  return plan_spin_tables(p, spin);
+}
+
+extern "C" int ga_alm2map_f64(const ga_plan *p, const double *alm, double *map, void *ws,
+                              size_t ws_bytes, void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(alm && map && ws, GA_ERR_INVALID, "NULL buffer");
+  GA_REQUIRE(ws_bytes >= ring_bytes(p, 1, 8), GA_ERR_WORKSPACE,
+             "workspace %zu < %zu bytes", ws_bytes, ring_bytes(p, 1, 8));
+  hipStream_t s = (hipStream_t)stream;
+  double *F = (double *)ws;
+  int rc = launch_legendre_synth_f64(p, alm, F, s);
+  if (rc) return rc;
+  return launch_ring_synth_f64(p, 1, F, map, nullptr, s);
+}
+
+extern "C" int ga_alm2map_batch_f32(const ga_plan *p, int32_t S, const float *alm, float *maps,
+                                    const ga_epilogue *epi, void *ws, size_t ws_bytes,
+                                    void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && alm && maps && ws, GA_ERR_INVALID, "invalid argument");
+  GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE,
+             "workspace %zu < %zu bytes", ws_bytes, ring_bytes(p, S, 4));
+  hipStream_t s = (hipStream_t)stream;
+  float *F = (float *)ws;
+  int rc = launch_legendre_synth_mfma(p, S, alm, F, s);
+  if (rc) return rc;
+  return launch_ring_synth_f32(p, S, F, maps, epi, s);
+}
+
+extern "C" int ga_legendre_synth_batch_f32(const ga_plan *p, int32_t S, const float *alm,
+                                           float *F, void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && alm && F, GA_ERR_INVALID, "invalid argument");
+  return launch_legendre_synth_mfma(p, S, alm, F, (hipStream_t)stream);
+}
+
+extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float *F, float *maps,
+                                       const ga_epilogue *epi, void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && F && maps, GA_ERR_INVALID, "invalid argument");
+  return launch_ring_synth_f32(p, S, F, maps, epi, (hipStream_t)stream);
+}
+
+template <typename T>
+static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter, T *map_tmp,
+                        void *ws, size_t ws_bytes, hipStream_t s) {
+  const int rb = int(sizeof(T));
+  GA_REQUIRE(map && alm && ws && niter >= 0 && (niter == 0 || map_tmp), GA_ERR_INVALID,
+             "invalid argument");
+  const size_t need = ring_bytes(p, S, rb) + partial_bytes(p, S, rb);
+  GA_REQUIRE(ws_bytes >= need, GA_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
+  T *F = (T *)ws;
+  T *partial = (T *)((char *)ws + ring_bytes(p, S, rb));
+  const int64_t npx = p->npix * S;
+  int rc;
+  for (int it = 0; it <= niter; ++it) {
+    const T *src = map;
+    if (it > 0) {
+      // map_tmp = map - S(alm)
+      if (sizeof(T) == 8) {
+        rc = launch_legendre_synth_f64(p, (const double *)alm, (double *)F, s);
+        if (rc) return rc;
+        rc = launch_ring_synth_f64(p, 1, (const double *)F, (double *)map_tmp, nullptr, s);
+      } else {
+        rc = launch_legendre_synth_mfma(p, S, (const float *)alm, (float *)F, s);
+        if (rc) return rc;
+        rc = launch_ring_synth_f32(p, S, (const float *)F, (float *)map_tmp, nullptr, s);
+      }
+      if (rc) return rc;
+      hipLaunchKernelGGL(residual_kernel<T>, dim3(unsigned((npx + 255) / 256)), dim3(256), 0, s,
+                         npx, map, map_tmp);
+      GA_LAUNCH_CHECK();
+      src = map_tmp;
+    }
+    if (sizeof(T) == 8) {
+      rc = launch_ring_anal_f64(p, S, (const double *)src, (double *)F, s);
+      if (rc) return rc;
+      rc = launch_legendre_anal_f64(p, S, (const double *)F, (double *)alm, (double *)partial,
+                                    it > 0, s);
+    } else {
+      rc = launch_ring_anal_f32(p, S, (const float *)src, (float *)F, s);
+      if (rc) return rc;
+      rc = launch_legendre_anal_f32(p, S, (const float *)F, (float *)alm, (float *)partial, it > 0,
+                                    s);
+    }
+    if (rc) return rc;
+  }
+  return GA_OK;
+}
+
+extern "C" int ga_map2alm_f64(const ga_plan *p, const double *map, double *alm, int32_t niter,
+                              double *map_tmp, void *ws, size_t ws_bytes, void *stream) {
+  GA_PLAN_OK(p);
+  return map2alm_impl<double>(p, 1, map, alm, niter, map_tmp, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int ga_map2alm_batch_f32(const ga_plan *p, int32_t S, const float *maps, float *alm,
+                                    int32_t niter, float *map_tmp, void *ws, size_t ws_bytes,
+                                    void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1, GA_ERR_INVALID, "nshell must be >= 1");
+  return map2alm_impl<float>(p, S, maps, alm, niter, map_tmp, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int ga_alm2map_spin_f64(const ga_plan *p, int32_t spin, const double *E,
+                                   const double *B, double *maps, void *ws, size_t ws_bytes,
+                                   void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(E && B && maps && ws, GA_ERR_INVALID, "NULL buffer");
+  GA_REQUIRE(ws_bytes >= ring_bytes(p, 2, 8), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
+             ws_bytes, ring_bytes(p, 2, 8));
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_spin_synth_f64(p, spin, 1, E, B, (double *)ws, s);
+  if (rc) return rc;
+  return launch_ring_synth_f64(p, 2, (const double *)ws, maps, nullptr, s);
+}
+
+extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t S,
+                                         const float *E, const float *B, float *maps, void *ws,
+                                         size_t ws_bytes, void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && E && B && maps && ws, GA_ERR_INVALID, "invalid argument");
+  GA_REQUIRE(ws_bytes >= ring_bytes(p, 2 * S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
+             ws_bytes, ring_bytes(p, 2 * S, 4));
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_spin_synth_f32(p, spin, S, E, B, (float *)ws, s);
+  if (rc) return rc;
+  return launch_ring_synth_f32(p, 2 * S, (const float *)ws, maps, nullptr, s);
+}

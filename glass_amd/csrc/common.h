@@ -1,0 +1,87 @@
+// Shared host/device helpers for the glass_amd HIP library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "glass_amd.h"
+
+namespace ga {
+
+// ---- error reporting ------------------------------------------------------
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+#define GA_CHECK_HIP(expr)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::ga::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                      __FILE__, __LINE__);                                     \
+      return GA_ERR_HIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+#define GA_REQUIRE(cond, code, ...)                                            \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::ga::set_error(__VA_ARGS__);                                            \
+      return (code);                                                           \
+    }                                                                          \
+  } while (0)
+
+#define GA_LAUNCH_CHECK()                                                      \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) {                                                    \
+      ::ga::set_error("kernel launch failed: %s (%s:%d)",                      \
+                      hipGetErrorString(e_), __FILE__, __LINE__);              \
+      return GA_ERR_HIP;                                                       \
+    }                                                                          \
+  } while (0)
+
+// ---- sizes ----------------------------------------------------------------
+inline int64_t nalm_of(int64_t lmax) { return (lmax + 1) * (lmax + 2) / 2; }
+inline int64_t npix_of(int64_t nside) { return 12 * nside * nside; }
+inline int64_t nrings_of(int64_t nside) { return 4 * nside - 1; }
+__host__ __device__ inline int64_t hp_index(int64_t lmax, int64_t l, int64_t m) {
+  return m * (2 * lmax + 1 - m) / 2 + l;
+}
+
+}  // namespace ga
+
+// ---- the plan (device tables for one (nside, lmax)) -----------------------
+// Ring pair rp = 0 .. 2 nside - 1 pairs north ring rp (0-based, north to south)
+// with south ring nrings - 1 - rp; rp = 2 nside - 1 is the equator (no partner).
+struct ga_plan {
+  int32_t nside, lmax, nrings, npairs;
+  int64_t npix, nalm;
+  int device;
+  // per ring pair (device): cos theta and sin theta of the north ring
+  double *d_z = nullptr, *d_sth = nullptr;
+  // per ring (device): first pixel, pixel count, phi0 = 2 pi pnum / pden
+  int64_t *d_ring_start = nullptr;
+  int32_t *d_ring_n = nullptr, *d_ring_pnum = nullptr, *d_ring_pden = nullptr;
+  // Legendre: recurrence coefficients (a, b) per (l, m) in alm order (+4 pad),
+  // lambda_l = a (x lambda_{l-1} - b lambda_{l-2}); seed constants c_m with
+  // lambda_mm = c_m sin^m theta
+  double2 *d_ab = nullptr;
+  double *d_cm = nullptr;
+  // spin recurrence coefficients (A, B, C) per (l, m) for s = 1, 2 (lazily built)
+  double *d_spin[3] = {nullptr, nullptr, nullptr};
+  // spin seed constants per m: [+s: (signed mantissa, exponent)] then [-s: ...]
+  double2 *d_spin_seed[3] = {nullptr, nullptr, nullptr};
+  // ring FFT: sub-length r per ring (n = 4 r), Bluestein tables
+  // per distinct non-power-of-two r: chirp exp(+i pi k^2 / r) (r entries) and
+  // the bit-reversed transform of its conjugate, scaled by 1 / L (L entries).
+  int32_t max_r = 0, max_L = 0;
+  int64_t *d_blu_off = nullptr;  // per r (0..max_r): offset into chirp tables, -1 if pow2
+  double2 *d_blu64 = nullptr;
+  float2 *d_blu32 = nullptr;
+  double2 *d_tw64 = nullptr;  // exp(2 pi i j / max_L), j < max_L
+  float2 *d_tw32 = nullptr;
+  size_t table_bytes = 0;
+};

@@ -1,0 +1,321 @@
+// Field-side kernels of the generation path: the conditional-Gaussian scaling
+// vectors (iternorm), the correlated alm of each shell (multalm sum, GLASS ->
+// healpy reorder and the m = 0 fix fused into one pass), almxfl, and the
+// pointwise transforms.  All of it is HBM-bound elementwise work.
+#include <cmath>
+
+#include "common.h"
+#include "device.h"
+#include "kernels.h"
+
+namespace ga {
+
+namespace {
+
+// healpy index i -> (l, m) for mmax = lmax = L:  i = m (2L + 1 - m) / 2 + l
+__device__ __forceinline__ void hp_decode(int64_t i, int L, int &l, int &m) {
+  const double b = 2.0 * L + 1.0;
+  int mm = int((b - sqrt(b * b - 8.0 * double(i))) * 0.5);
+  if (mm < 0) mm = 0;
+  while (mm > 0 && hp_index(L, mm, mm) > i) --mm;
+  while (mm < L && hp_index(L, mm + 1, mm + 1) <= i) ++mm;
+  m = mm;
+  l = int(i - hp_index(L, mm, mm)) + mm;
+}
+
+// GLASS index g = l (l + 1) / 2 + m -> l
+__device__ __forceinline__ int glass_l(int64_t g) {
+  int l = int((sqrt(8.0 * double(g) + 1.0) - 1.0) * 0.5);
+  while (int64_t(l) * (l + 1) / 2 > g) --l;
+  while (int64_t(l + 1) * (l + 2) / 2 <= g) ++l;
+  return l;
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011), counter-based
+struct u32x4 {
+  uint32_t v[4];
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c.v[0];
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.v[2];
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    c = u32x4{{hi1 ^ c.v[1] ^ k0, lo1, hi0 ^ c.v[3] ^ k1, lo0}};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// standard complex normal pair (re, im) for GLASS-order index g of shell t:
+// Box-Muller on two 53-bit uniforms built from the four Philox words
+__device__ __forceinline__ double2 device_normal_pair(uint64_t seed, int64_t t, int64_t g) {
+  u32x4 c{{uint32_t(g), uint32_t(uint64_t(g) >> 32), uint32_t(t), uint32_t(uint64_t(t) >> 32)}};
+  const u32x4 r = philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+  const uint64_t a = (uint64_t(r.v[0]) << 21) ^ (uint64_t(r.v[1]) >> 11);  // 53 bits
+  const uint64_t b = (uint64_t(r.v[2]) << 21) ^ (uint64_t(r.v[3]) >> 11);
+  const double u1 = (double(a & ((uint64_t(1) << 53) - 1)) + 0.5) * 0x1p-53;  // (0, 1)
+  const double u2 = double(b & ((uint64_t(1) << 53) - 1)) * 0x1p-53;
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, co;
+  sincospi(2.0 * u2, &s, &co);
+  return double2{rad * co, rad * s};
+}
+
+// State per mode l (capacity kmax): M [kmax][kmax] row-major, a [kmax], s,
+// then kmax doubles of scratch.
+// Row t of this call has k + 1 entries; the state entering it has size kprev
+// (= k for every row after the first of a call).  Update (arXiv:2302.01942
+// App. A): grow M to (kprev+1)^2 with the new bottom row [-a^T M, u] / s'
+// (u = 1, s' = s where s > 0; u = 0, s' = 1 otherwise), keep the trailing
+// k x k block, then a = M c (c = correlations oldest first), s^2 = C_0 - a.a.
+__global__ void iternorm_kernel(int nl, int kprev0, int k, int kmax, int nshell, int start,
+                                const double *__restrict__ cov, double *__restrict__ state,
+                                double *__restrict__ w, int *__restrict__ flag) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  double *M = state + int64_t(l) * (kmax * kmax + 2 * kmax + 1);
+  double *a = M + kmax * kmax;
+  double *s = a + kmax;
+  double *atm = s + 1;
+  int kprev = kprev0;
+  for (int t = 0; t < nshell; ++t) {
+    const double *row = cov + (int64_t(t) * nl + l) * (k + 1);
+    double *wt = w + (int64_t(t) * nl + l) * (k + 1);
+    if (t == 0 && start) {
+      for (int i = 0; i < k; ++i) {
+        for (int j = 0; j < k; ++j) M[i * kmax + j] = 0.0;
+        a[i] = 0.0;
+      }
+      *s = 1.0;
+    } else {
+      const double sv = *s;
+      const double u = sv > 0.0 ? 1.0 : 0.0;
+      const double sp = sv > 0.0 ? sv : 1.0;
+      // atm <- a^T M / s'
+      for (int j = 0; j < kprev; ++j) {
+        double acc = 0.0;
+        for (int i = 0; i < kprev; ++i) acc += a[i] * M[i * kmax + j];
+        atm[j] = acc / sp;
+      }
+      const int d = kprev + 1 - k;  // rows/cols dropped from the grown matrix
+      for (int i = 0; i < k; ++i) {
+        const int bi = i + d;
+        for (int j = 0; j < k; ++j) {
+          const int bj = j + d;
+          double v;
+          if (bi < kprev) v = (bj < kprev) ? M[bi * kmax + bj] : 0.0;
+          else v = (bj < kprev) ? -atm[bj] : u / sp;
+          M[i * kmax + j] = v;
+        }
+      }
+    }
+    double aa = 0.0;
+    for (int i = 0; i < k; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < k; ++j) acc += M[i * kmax + j] * row[k - j];
+      a[i] = acc;
+      aa += acc * acc;
+    }
+    const double var = row[0] - aa;
+    if (var < 0.0) atomicOr(flag, 1);
+    *s = sqrt(var > 0.0 ? var : 0.0);
+    for (int i = 0; i < k; ++i) wt[i] = a[i];
+    wt[k] = *s;
+    kprev = k;
+  }
+}
+
+// One thread per (healpy index, shell); shell index innermost for coalescing.
+template <bool HOSTZ>
+__global__ void fields_alm_kernel(int nl, int k, int first_shell, int nshell, int64_t nalm,
+                                  const double *__restrict__ w, const double2 *__restrict__ zring,
+                                  int nslot, uint64_t seed, float2 *__restrict__ out32,
+                                  double2 *__restrict__ out64) {
+  const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= nalm * nshell) return;
+  const int64_t i = gid / nshell;
+  const int s = int(gid - i * nshell);
+  int l, m;
+  hp_decode(i, nl - 1, l, m);
+  const int64_t g = int64_t(l) * (l + 1) / 2 + m;
+  const int64_t z_size = int64_t(nl) * (nl + 1) / 2;
+  const int64_t t = int64_t(first_shell) + s;
+  const int len = int(t + 1 < k + 1 ? t + 1 : k + 1);
+  const int mis = k + 1 - len;
+  const double *wl = w + (int64_t(s) * nl + l) * (k + 1);
+  double re = 0.0, im = 0.0;
+  for (int j = mis; j <= k; ++j) {
+    const int64_t tz = t - k + j;  // shell whose normals enter with weight w[j]
+    double2 z;
+    if (HOSTZ)
+      z = zring[(tz % nslot) * z_size + g];
+    else
+      z = device_normal_pair(seed, tz, g);
+    re = fma(wl[j], z.x, re);
+    im = fma(wl[j], z.y, im);
+  }
+  if (m == 0) { re += im; im = 0.0; }
+  if (out32) out32[gid] = float2{float(re), float(im)};
+  else out64[i] = double2{re, im};
+}
+
+__global__ void multalm_kernel(int64_t n, const double2 *__restrict__ alm,
+                               const double *__restrict__ bl, double2 *__restrict__ out) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double f = bl[glass_l(g)];
+  const double2 a = alm[g];
+  out[g] = double2{a.x * f, a.y * f};
+}
+
+__global__ void glass_to_healpix_kernel(int nl, int64_t n, const double2 *__restrict__ in,
+                                        double2 *__restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int l, m;
+  hp_decode(i, nl - 1, l, m);
+  out[i] = in[int64_t(l) * (l + 1) / 2 + m];
+}
+
+template <typename T>
+__global__ void almxfl_kernel(int lmax, int nshell, int64_t n, const T *__restrict__ alm,
+                              const double *__restrict__ fl, int nfl, T *__restrict__ out) {
+  const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n) return;
+  const int64_t i = gid / nshell;
+  int l, m;
+  hp_decode(i, lmax, l, m);
+  const double f = l < nfl ? fl[l] : 0.0;
+  const T a = alm[gid];
+  out[gid] = T{decltype(a.x)(a.x * f), decltype(a.y)(a.y * f)};
+}
+
+__global__ void transform_f64_kernel(int64_t n, const double *__restrict__ x, int kind,
+                                     double lam, double par, double *__restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = x[i];
+  if (kind == GA_T_LOGNORMAL) {
+    v = expm1(v - par * 0.5);
+    if (lam != 1.0) v *= lam;
+  } else if (kind == GA_T_SQUARED_NORMAL) {
+    v = (v - par) * (v - par) - 1.0;
+    if (lam != 1.0) v *= lam;
+  }
+  out[i] = v;
+}
+
+inline unsigned nblocks(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+}  // namespace
+
+}  // namespace ga
+
+using namespace ga;
+
+extern "C" int ga_iternorm_state_doubles(int32_t kmax, int32_t *n) {
+  GA_REQUIRE(n && kmax >= 0, GA_ERR_INVALID, "invalid argument");
+  *n = kmax * kmax + 2 * kmax + 1;
+  return GA_OK;
+}
+
+extern "C" int ga_iternorm_batch(int32_t nl, int32_t kprev, int32_t k, int32_t kmax,
+                                 int32_t nshell, int32_t start, const double *cov, double *state,
+                                 double *w, int32_t *flag, void *stream) {
+  clear_error();
+  GA_REQUIRE(nl > 0 && k >= 0 && kmax >= k && nshell > 0 && cov && state && w && flag,
+             GA_ERR_INVALID, "invalid argument to ga_iternorm_batch");
+  GA_REQUIRE(start || (kprev >= 0 && kprev <= kmax && k <= kprev + 1), GA_ERR_INVALID,
+             "shape mismatch in covariance (%d correlations after %d)", k, kprev);
+  hipLaunchKernelGGL(iternorm_kernel, dim3(nblocks(nl, 128)), dim3(128), 0, (hipStream_t)stream,
+                     nl, kprev, k, kmax, nshell, start, cov, state, w, flag);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
+                             const double *w, const double *zring, int32_t nslot, uint64_t seed,
+                             float *out_f32, double *out_f64, void *stream) {
+  clear_error();
+  GA_REQUIRE(nl > 0 && k >= 0 && first_shell >= 0 && nshell > 0 && w, GA_ERR_INVALID,
+             "invalid argument to ga_fields_alm");
+  GA_REQUIRE((out_f32 != nullptr) != (out_f64 != nullptr), GA_ERR_INVALID,
+             "exactly one of out_f32 / out_f64 must be given");
+  GA_REQUIRE(out_f64 == nullptr || nshell == 1, GA_ERR_INVALID, "out_f64 requires nshell == 1");
+  GA_REQUIRE(zring == nullptr || nslot >= nshell + k, GA_ERR_INVALID,
+             "normal ring buffer needs nslot >= nshell + k (%d < %d)", nslot, nshell + k);
+  const int64_t nalm = int64_t(nl) * (nl + 1) / 2;
+  const int64_t n = nalm * nshell;
+  const dim3 grid(nblocks(n, 256)), block(256);
+  if (zring)
+    hipLaunchKernelGGL(fields_alm_kernel<true>, grid, block, 0, (hipStream_t)stream, nl, k,
+                       first_shell, nshell, nalm, w, reinterpret_cast<const double2 *>(zring),
+                       nslot, seed, reinterpret_cast<float2 *>(out_f32),
+                       reinterpret_cast<double2 *>(out_f64));
+  else
+    hipLaunchKernelGGL(fields_alm_kernel<false>, grid, block, 0, (hipStream_t)stream, nl, k,
+                       first_shell, nshell, nalm, w, nullptr, 1, seed,
+                       reinterpret_cast<float2 *>(out_f32), reinterpret_cast<double2 *>(out_f64));
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+extern "C" int ga_multalm_f64(int32_t nl, const double *alm, const double *bl, double *out,
+                              void *stream) {
+  clear_error();
+  GA_REQUIRE(nl >= 0 && (nl == 0 || (alm && bl && out)), GA_ERR_INVALID, "invalid argument");
+  const int64_t n = int64_t(nl) * (nl + 1) / 2;
+  if (n == 0) return GA_OK;
+  hipLaunchKernelGGL(multalm_kernel, dim3(nblocks(n, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                     reinterpret_cast<const double2 *>(alm), bl, reinterpret_cast<double2 *>(out));
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+extern "C" int ga_glass_to_healpix_alm_f64(int32_t nl, const double *alm_glass, double *alm_hp,
+                                           void *stream) {
+  clear_error();
+  GA_REQUIRE(nl > 0 && alm_glass && alm_hp && alm_glass != alm_hp, GA_ERR_INVALID,
+             "invalid argument (in-place reorder is not supported)");
+  const int64_t n = int64_t(nl) * (nl + 1) / 2;
+  hipLaunchKernelGGL(glass_to_healpix_kernel, dim3(nblocks(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, nl, n, reinterpret_cast<const double2 *>(alm_glass),
+                     reinterpret_cast<double2 *>(alm_hp));
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+extern "C" int ga_almxfl(int32_t lmax, int32_t nshell, int32_t real_bytes, const void *alm,
+                         const double *fl, int32_t nfl, void *out, void *stream) {
+  clear_error();
+  GA_REQUIRE(lmax >= 0 && nshell > 0 && alm && out && (fl || nfl == 0) && nfl >= 0,
+             GA_ERR_INVALID, "invalid argument to ga_almxfl");
+  const int64_t n = nalm_of(lmax) * nshell;
+  if (real_bytes == 8)
+    hipLaunchKernelGGL(almxfl_kernel<double2>, dim3(nblocks(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, lmax, nshell, n, (const double2 *)alm, fl, nfl,
+                       (double2 *)out);
+  else if (real_bytes == 4)
+    hipLaunchKernelGGL(almxfl_kernel<float2>, dim3(nblocks(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, lmax, nshell, n, (const float2 *)alm, fl, nfl,
+                       (float2 *)out);
+  else
+    GA_REQUIRE(false, GA_ERR_INVALID, "real_bytes must be 4 or 8");
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+extern "C" int ga_transform_f64(int64_t n, const double *x, const ga_epilogue *epi, double *out,
+                                void *stream) {
+  clear_error();
+  GA_REQUIRE(n >= 0 && epi && (n == 0 || (x && out)), GA_ERR_INVALID, "invalid argument");
+  if (n == 0) return GA_OK;
+  hipLaunchKernelGGL(transform_f64_kernel, dim3(nblocks(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, x, epi->kind, epi->lamda, epi->param, out);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}

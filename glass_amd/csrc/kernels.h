@@ -1,0 +1,41 @@
+// Host-side launchers for the HIP kernels (internal to the library).
+#pragma once
+
+#include "common.h"
+
+namespace ga {
+
+// Legendre stage, FP64 plain: alm (healpy order complex128) -> F [ring][m] complex128
+int launch_legendre_synth_f64(const ga_plan *p, const double *alm, double *F, hipStream_t s);
+
+// Legendre stage, FP32 MFMA batched: alm [idx][S] complex64 -> F [ring][m][S] complex64
+int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float *F, hipStream_t s);
+
+// Ring stage of synthesis: F [ring][m][S] -> maps [S][npix] with fused transform
+int launch_ring_synth_f64(const ga_plan *p, int S, const double *F, double *maps,
+                          const ga_epilogue *epi, hipStream_t s);
+int launch_ring_synth_f32(const ga_plan *p, int S, const float *F, float *maps,
+                          const ga_epilogue *epi, hipStream_t s);
+
+// Ring stage of analysis: maps [S][npix] -> F [ring][m][S] (weighted, phase removed)
+int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s);
+int launch_ring_anal_f32(const ga_plan *p, int S, const float *maps, float *F, hipStream_t s);
+
+// Legendre stage of analysis: F [ring][m][S] -> alm [idx][S] (+= when accumulate)
+// partial: anal_partial_slots(p) * nalm * S complex scratch
+int anal_partial_slots(const ga_plan *p);
+int launch_legendre_anal_f64(const ga_plan *p, int S, const double *F, double *alm,
+                             double *partial, bool accumulate, hipStream_t s);
+int launch_legendre_anal_f32(const ga_plan *p, int S, const float *F, float *alm, float *partial,
+                             bool accumulate, hipStream_t s);
+
+// Spin synthesis Legendre stage: (E, B) [idx][S] -> ring buffer [ring][m][2S],
+// column 2s = Q_s, 2s + 1 = U_s
+int launch_spin_synth_f64(const ga_plan *p, int spin, int S, const double *E, const double *B,
+                          double *Fo, hipStream_t s);
+int launch_spin_synth_f32(const ga_plan *p, int spin, int S, const float *E, const float *B,
+                          float *Fo, hipStream_t s);
+
+int plan_spin_tables(ga_plan *p, int s);
+
+}  // namespace ga

@@ -1,0 +1,331 @@
+// Plan creation: HEALPix RING geometry, Legendre recurrence tables and ring-FFT
+// tables for one (nside, lmax), computed on the host in FP64 and uploaded once.
+//
+// Geometry follows Gorski et al. 2005 (ApJ 622, 759), section 4: north polar
+// cap rings i < nside hold 4 i pixels at z = 1 - i^2 / (3 nside^2) with
+// phi0 = pi / (4 i); equatorial rings nside <= i <= 3 nside hold 4 nside pixels
+// at z = 4/3 - 2 i / (3 nside) with phi0 = pi / (4 nside) when i + nside is even
+// and 0 otherwise; the south cap mirrors the north.
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+namespace ga {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+
+namespace {
+
+struct RingGeom {
+  int64_t start;
+  int32_t n, pnum, pden;
+  double z, sth;
+};
+
+RingGeom ring_geom(int64_t nside, int64_t r /* 1-based */) {
+  RingGeom g{};
+  const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+  if (r < nside) {
+    double t = double(r * r) / (3.0 * double(nside * nside));
+    g.n = int32_t(4 * r);
+    g.start = 2 * r * (r - 1);
+    g.z = 1.0 - t;
+    g.sth = std::sqrt(t * (2.0 - t));
+    g.pden = int32_t(8 * r);
+    g.pnum = 1;
+  } else if (r <= 3 * nside) {
+    g.n = int32_t(4 * nside);
+    g.start = ncap + (r - nside) * 4 * nside;
+    g.z = double(2 * nside - r) * 2.0 / (3.0 * double(nside));
+    g.sth = std::sqrt((1.0 - g.z) * (1.0 + g.z));
+    g.pden = int32_t(8 * nside);
+    g.pnum = ((r + nside) & 1) ? 0 : 1;
+  } else {
+    int64_t i = 4 * nside - r;
+    double t = double(i * i) / (3.0 * double(nside * nside));
+    g.n = int32_t(4 * i);
+    g.start = npix - 2 * i * (i + 1);
+    g.z = -(1.0 - t);
+    g.sth = std::sqrt(t * (2.0 - t));
+    g.pden = int32_t(8 * i);
+    g.pnum = 1;
+  }
+  return g;
+}
+
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+
+int64_t blu_len(int64_t r) {
+  int64_t L = 1;
+  while (L < 2 * r - 1) L <<= 1;
+  return L;
+}
+
+unsigned bitrev(unsigned x, int bits) {
+  unsigned y = 0;
+  for (int i = 0; i < bits; ++i) { y = (y << 1) | (x & 1u); x >>= 1; }
+  return y;
+}
+
+// in-place forward complex FFT (exp(-2 pi i jk / n)), n power of two
+void host_fft(std::vector<std::complex<double>> &a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t k = 0; k < len / 2; ++k) {
+      double ang = -2.0 * M_PI * double(k) / double(len);
+      std::complex<double> w(std::cos(ang), std::sin(ang));
+      for (size_t s = 0; s < n; s += len) {
+        auto u = a[s + k], v = a[s + k + len / 2] * w;
+        a[s + k] = u + v;
+        a[s + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+template <typename T>
+int upload(T **dst, const void *src, size_t bytes, size_t *acc) {
+  if (bytes == 0) return GA_OK;
+  hipError_t e = hipMalloc((void **)dst, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) for plan tables failed: %s", bytes, hipGetErrorString(e));
+    return GA_ERR_NOMEM;
+  }
+  GA_CHECK_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  *acc += bytes;
+  return GA_OK;
+}
+
+}  // namespace
+
+// Spin recurrence tables, built on first use: per (l, m) in alm order,
+// (A, B, C) with  sl_{l} = A (x + B) sl_{l-1} - C sl_{l-2}  (+s) and
+// (x - B) for -s, from the Wigner-d three-term recurrence.
+int plan_spin_tables(ga_plan *p, int s) {
+  if (p->d_spin[s]) return GA_OK;
+  const int64_t lmax = p->lmax, nalm = p->nalm;
+  std::vector<double> t(size_t(3 * (nalm + 4)), 0.0);
+  std::vector<double2> seeds(size_t(2 * (lmax + 1)), double2{0.0, 0.0});
+  for (int64_t m = 0; m <= lmax; ++m) {
+    int64_t l0 = m > s ? m : s;
+    // (0, 0, -1) at l = l0: one step from (seed, 0) yields the seed itself
+    if (l0 <= lmax) {
+      size_t i0 = size_t(3 * hp_index(lmax, l0, m));
+      t[i0] = 0.0; t[i0 + 1] = 0.0; t[i0 + 2] = -1.0;
+    }
+    // |seed constant| = sqrt((2 l0 + 1)/(4 pi) (2 l0)! / ((m + s)! |m - s|!)), as m 2^e
+    {
+      const int64_t dif = m > s ? m - s : s - m;
+      long double l2 = 0.5L * (std::log2((2.0L * l0 + 1.0L) / (4.0L * (long double)M_PI)) +
+                               (std::lgamma((long double)(2 * l0 + 1)) -
+                                std::lgamma((long double)(m + s + 1)) -
+                                std::lgamma((long double)(dif + 1))) /
+                                   std::log(2.0L));
+      long double e = std::floor(l2);
+      double mant = double(std::exp2(l2 - e));
+      double sp = (m & 1) ? -1.0 : 1.0;
+      double sm = (m >= s) ? sp : ((s & 1) ? -1.0 : 1.0);
+      seeds[size_t(m)] = double2{sp * mant, double(e)};
+      seeds[size_t(lmax + 1 + m)] = double2{sm * mant, double(e)};
+    }
+    for (int64_t l = l0 + 1; l <= lmax; ++l) {
+      double L = double(l - 1), mm = double(m), ss = double(s);
+      double den = std::sqrt(((L + 1) * (L + 1) - mm * mm) * ((L + 1) * (L + 1) - ss * ss));
+      double A = std::sqrt((2 * L + 3) * (2 * L + 1)) * (L + 1) / den;
+      double B = mm * ss / (L * (L + 1));
+      double C = std::sqrt(2 * L + 3) * (L + 1) * std::sqrt((L * L - mm * mm) * (L * L - ss * ss)) /
+                 (L * std::sqrt(2 * L - 1) * den);
+      size_t i = size_t(3 * hp_index(lmax, l, m));
+      t[i] = A; t[i + 1] = B; t[i + 2] = C;
+    }
+  }
+  int rc = upload(&p->d_spin[s], t.data(), t.size() * sizeof(double), &p->table_bytes);
+  if (rc) return rc;
+  return upload(&p->d_spin_seed[s], seeds.data(), seeds.size() * sizeof(double2), &p->table_bytes);
+}
+
+}  // namespace ga
+
+using namespace ga;
+
+extern "C" const char *ga_last_error(void) { return g_err.c_str(); }
+extern "C" const char *ga_version(void) { return "glass_amd 0.1 (gfx950)"; }
+
+extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
+  clear_error();
+  GA_REQUIRE(out != nullptr, GA_ERR_INVALID, "plan output pointer is NULL");
+  GA_REQUIRE(nside >= 1 && nside <= 8192, GA_ERR_UNSUPPORTED, "nside %d outside 1..8192", nside);
+  GA_REQUIRE(lmax >= 0 && lmax <= 16384, GA_ERR_UNSUPPORTED, "lmax %d outside 0..16384", lmax);
+  ga_plan *p = new (std::nothrow) ga_plan();
+  GA_REQUIRE(p != nullptr, GA_ERR_NOMEM, "host allocation of plan failed");
+  p->nside = nside;
+  p->lmax = lmax;
+  p->nrings = int32_t(nrings_of(nside));
+  p->npairs = 2 * nside;
+  p->npix = npix_of(nside);
+  p->nalm = nalm_of(lmax);
+  (void)hipGetDevice(&p->device);
+  int rc = GA_OK;
+  auto fail = [&](int code) { ga_plan_destroy(p); return code; };
+
+  // ---- geometry
+  std::vector<double> z(size_t(p->npairs)), sth(size_t(p->npairs));
+  for (int rp = 0; rp < p->npairs; ++rp) {
+    RingGeom g = ring_geom(nside, rp + 1);
+    z[size_t(rp)] = g.z;
+    sth[size_t(rp)] = g.sth;
+  }
+  std::vector<int64_t> start(size_t(p->nrings));
+  std::vector<int32_t> rn(size_t(p->nrings)), pnum(size_t(p->nrings)), pden(size_t(p->nrings));
+  for (int r = 0; r < p->nrings; ++r) {
+    RingGeom g = ring_geom(nside, r + 1);
+    start[size_t(r)] = g.start;
+    rn[size_t(r)] = g.n;
+    pnum[size_t(r)] = g.pnum;
+    pden[size_t(r)] = g.pden;
+  }
+  if ((rc = upload(&p->d_z, z.data(), z.size() * 8, &p->table_bytes))) return fail(rc);
+  if ((rc = upload(&p->d_sth, sth.data(), sth.size() * 8, &p->table_bytes))) return fail(rc);
+  if ((rc = upload(&p->d_ring_start, start.data(), start.size() * 8, &p->table_bytes))) return fail(rc);
+  if ((rc = upload(&p->d_ring_n, rn.data(), rn.size() * 4, &p->table_bytes))) return fail(rc);
+  if ((rc = upload(&p->d_ring_pnum, pnum.data(), pnum.size() * 4, &p->table_bytes))) return fail(rc);
+  if ((rc = upload(&p->d_ring_pden, pden.data(), pden.size() * 4, &p->table_bytes))) return fail(rc);
+
+  // ---- Legendre tables
+  {
+    std::vector<double2> ab(size_t(p->nalm + 4), double2{0.0, 0.0});
+    for (int64_t m = 0; m <= lmax; ++m) {
+      const double mm = double(m);
+      // (1, -1) at l = m: one step from (lambda_{m-2}, lambda_{m-1}) = (seed, 0)
+      // yields lambda_mm, so kernels can step uniformly from l = m
+      ab[size_t(hp_index(lmax, m, m))] = double2{1.0, -1.0};
+      for (int64_t l = m + 1; l <= lmax; ++l) {
+        const double ll = double(l);
+        double a = std::sqrt((4.0 * ll * ll - 1.0) / (ll * ll - mm * mm));
+        double b = std::sqrt(((ll - 1.0) * (ll - 1.0) - mm * mm) / (4.0 * (ll - 1.0) * (ll - 1.0) - 1.0));
+        ab[size_t(hp_index(lmax, l, m))] = double2{a, b};
+      }
+    }
+    if ((rc = upload(&p->d_ab, ab.data(), ab.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
+    // c_m = (-1)^m sqrt((2m+1)/(4 pi) prod_{k<=m} (2k-1)/(2k)): lambda_mm = c_m sin^m
+    std::vector<double> cm(size_t(lmax + 1));
+    long double acc = 1.0L;
+    for (int64_t m = 0; m <= lmax; ++m) {
+      if (m > 0) acc *= (2.0L * m - 1.0L) / (2.0L * m);
+      double v = double(std::sqrt((2.0L * m + 1.0L) / (4.0L * (long double)M_PI) * acc));
+      cm[size_t(m)] = (m & 1) ? -v : v;
+    }
+    if ((rc = upload(&p->d_cm, cm.data(), cm.size() * 8, &p->table_bytes))) return fail(rc);
+  }
+
+  // ---- ring FFT tables: every ring has n = 4 r pixels
+  {
+    int32_t max_r = 0;
+    for (int r = 0; r < p->nrings; ++r) max_r = std::max(max_r, rn[size_t(r)] / 4);
+    p->max_r = max_r;
+    std::vector<int64_t> off(size_t(max_r + 1), -1);
+    std::vector<char> used(size_t(max_r + 1), 0);
+    for (int r = 0; r < p->nrings; ++r) used[size_t(rn[size_t(r)] / 4)] = 1;
+    int64_t total = 0, maxL = 1;
+    for (int32_t r = 1; r <= max_r; ++r) {
+      if (!used[size_t(r)]) continue;
+      if (is_pow2(r)) {
+        maxL = std::max<int64_t>(maxL, r);
+        continue;
+      }
+      int64_t L = blu_len(r);
+      maxL = std::max(maxL, L);
+      off[size_t(r)] = total;
+      total += r + L;
+    }
+    p->max_L = int32_t(maxL);
+    std::vector<double2> blu(size_t(std::max<int64_t>(total, 1)));
+    for (int32_t r = 1; r <= max_r; ++r) {
+      if (off[size_t(r)] < 0) continue;
+      const int64_t L = blu_len(r);
+      int bits = 0;
+      while ((int64_t(1) << bits) < L) ++bits;
+      double2 *c = blu.data() + off[size_t(r)];
+      double2 *bh = c + r;
+      std::vector<std::complex<double>> b(size_t(L), 0.0);
+      for (int64_t k = 0; k < r; ++k) {
+        int64_t q = (k * k) % (2 * r);
+        double ang = M_PI * double(q) / double(r);
+        c[k] = double2{std::cos(ang), std::sin(ang)};
+        std::complex<double> cc(std::cos(ang), -std::sin(ang));
+        b[size_t(k)] = cc;
+        if (k > 0) b[size_t(L - k)] = cc;
+      }
+      host_fft(b);
+      for (int64_t i = 0; i < L; ++i) {
+        auto v = b[size_t(i)] / double(L);
+        bh[bitrev(unsigned(i), bits)] = double2{v.real(), v.imag()};
+      }
+    }
+    std::vector<float2> blu32(blu.size());
+    for (size_t i = 0; i < blu.size(); ++i) blu32[i] = float2{float(blu[i].x), float(blu[i].y)};
+    std::vector<double2> tw(static_cast<size_t>(maxL));
+    std::vector<float2> tw32(static_cast<size_t>(maxL));
+    for (int64_t j = 0; j < maxL; ++j) {
+      double ang = 2.0 * M_PI * double(j) / double(maxL);
+      tw[size_t(j)] = double2{std::cos(ang), std::sin(ang)};
+      tw32[size_t(j)] = float2{float(tw[size_t(j)].x), float(tw[size_t(j)].y)};
+    }
+    if ((rc = upload(&p->d_blu_off, off.data(), off.size() * 8, &p->table_bytes))) return fail(rc);
+    if ((rc = upload(&p->d_blu64, blu.data(), blu.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
+    if ((rc = upload(&p->d_blu32, blu32.data(), blu32.size() * sizeof(float2), &p->table_bytes))) return fail(rc);
+    if ((rc = upload(&p->d_tw64, tw.data(), tw.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
+    if ((rc = upload(&p->d_tw32, tw32.data(), tw32.size() * sizeof(float2), &p->table_bytes))) return fail(rc);
+  }
+  *out = p;
+  return GA_OK;
+}
+
+extern "C" int ga_plan_destroy(ga_plan *p) {
+  if (!p) return GA_OK;
+  void *ptrs[] = {p->d_z, p->d_sth, p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,
+                  p->d_ab, p->d_cm, p->d_spin[0], p->d_spin[1], p->d_spin[2],
+                  p->d_spin_seed[0], p->d_spin_seed[1], p->d_spin_seed[2], p->d_blu_off,
+                  p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  delete p;
+  return GA_OK;
+}
+
+extern "C" int ga_plan_table_bytes(const ga_plan *p, size_t *bytes) {
+  GA_REQUIRE(p && bytes, GA_ERR_INVALID, "NULL argument");
+  *bytes = p->table_bytes;
+  return GA_OK;
+}
+
+extern "C" int ga_workspace_size(const ga_plan *p, int32_t nbatch, int32_t real_bytes,
+                                 size_t *bytes) {
+  GA_REQUIRE(p && bytes, GA_ERR_INVALID, "NULL argument");
+  GA_REQUIRE(nbatch >= 1, GA_ERR_INVALID, "nbatch must be >= 1");
+  GA_REQUIRE(real_bytes == 4 || real_bytes == 8, GA_ERR_INVALID, "real_bytes must be 4 or 8");
+  *bytes = size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nbatch) * 2 * size_t(real_bytes);
+  return GA_OK;
+}

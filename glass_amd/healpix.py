@@ -1,0 +1,275 @@
+"""HEALPix transforms on MI355X, mirroring ``glass.healpix`` (glass/healpix.py).
+
+The reference wraps healpy with ``numpy_fallback`` (glass/_array_api_utils.py:
+569-652): arrays are converted to numpy, healpy runs on the host, results are
+converted back to the caller's namespace.  Here the same functions run on the
+GPU through the C ABI: numpy inputs come back as numpy arrays, torch tensors
+come back as device tensors.  ``precision="fp64"`` (default) is the plain FP64
+path; ``precision="fp32"`` runs the batched FP32-MFMA synthesis.
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import pathlib
+import warnings
+from collections.abc import Sequence
+
+import numpy as np
+
+from . import _lib
+
+__all__ = [
+    "alm2map", "alm2map_spin", "almxfl", "get_nside", "map2alm", "npix2nside",
+    "nside2npix", "pixwin",
+]
+
+
+def _get_healpy_datapath() -> str | None:
+    """glass/healpix.py:31-35."""
+    p = os.environ.get("HEALPY_DATAPATH")
+    if p is not None and not pathlib.Path(p).is_dir():
+        raise ValueError(f"Healpy datapath not found at '{p}'")
+    return p
+
+
+def nside2npix(nside: int) -> int:
+    """glass/healpix.py:296-310."""
+    return 12 * int(nside) * int(nside)
+
+
+def npix2nside(npix: int) -> int:
+    """glass/healpix.py:279-293."""
+    nside = math.isqrt(int(npix) // 12)
+    if 12 * nside * nside != int(npix):
+        raise ValueError(f"Wrong pixel number (it is not 12*nside**2): {npix}")
+    return nside
+
+
+def get_nside(m) -> int:
+    """glass/healpix.py:214-229: nside of a map (last axis)."""
+    return npix2nside(m.shape[-1])
+
+
+def nalm(lmax: int) -> int:
+    return (lmax + 1) * (lmax + 2) // 2
+
+
+def lmax_from_nalm(n: int) -> int:
+    lmax = (math.isqrt(8 * int(n) + 1) - 3) // 2
+    if nalm(lmax) != int(n):
+        raise ValueError(f"Wrong alm size (not a triangular number of lmax): {n}")
+    return lmax
+
+
+def _is_single(alms) -> bool:
+    return getattr(alms, "ndim", None) == 1
+
+
+def _as_list(alms):
+    if hasattr(alms, "ndim"):
+        return [alms[i] for i in range(alms.shape[0])] if alms.ndim == 2 else [alms]
+    return list(alms)
+
+
+def _synth_f64(alm_dev, nside: int, lmax: int):
+    T = _lib.torch()
+    plan = _lib.get_plan(nside, lmax)
+    ws = _lib.workspace(plan.workspace_bytes(1, 8))
+    out = T.empty(nside2npix(nside), dtype=T.float64, device="cuda")
+    _lib.call("ga_alm2map_f64", plan.h, _lib.ptr(alm_dev), _lib.ptr(out), _lib.ptr(ws),
+              ws.numel(), _lib.stream_ptr())
+    return out
+
+
+def synth_batch_f32(alm_batch, nside: int, lmax: int, epilogues=None, out=None):
+    """alm_batch: complex64 device tensor [nalm, S] (shell innermost) -> [S, npix] float32."""
+    T = _lib.torch()
+    S = int(alm_batch.shape[1])
+    plan = _lib.get_plan(nside, lmax)
+    ws = _lib.workspace(plan.workspace_bytes(S, 4))
+    if out is None:
+        out = T.empty((S, nside2npix(nside)), dtype=T.float32, device="cuda")
+    epi_ptr = _lib.ptr(epilogues) if epilogues is not None else None
+    _lib.call("ga_alm2map_batch_f32", plan.h, S, _lib.ptr(alm_batch), _lib.ptr(out), epi_ptr,
+              _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    return out
+
+
+def alm2map(alms, nside: int, *, inplace: bool = False, lmax: int | None = None,
+            pixwin: bool = False, pol: bool = True, precision: str = "fp64"):
+    """glass/healpix.py:38-78 (healpy.alm2map, RING output, mmax = lmax).
+
+    A single 1-D alm gives one map.  A sequence of three with ``pol=True`` is
+    read as (T, E, B) and gives [T, Q, U] (Q + iU = -sum (E + iB) 2Y_lm); any
+    other sequence gives one map per alm.
+    """
+    del inplace  # outputs are always fresh device buffers; inputs are never modified
+    if pixwin:
+        raise NotImplementedError(
+            "pixwin=True needs the HEALPix pixel-window tables (healpy data files), "
+            "which this build does not ship; apply the window with almxfl instead")
+    single = _is_single(alms)
+    lst = [alms] if single else _as_list(alms)
+    if pol and len(lst) == 3 and not single:
+        t = alm2map(lst[0], nside, lmax=lmax, pol=False, precision=precision)
+        q, u = alm2map_spin(lst[1:], nside, 2, lmax if lmax is not None else
+                            lmax_from_nalm(lst[1].shape[-1]), precision=precision)
+        return [t, q, u]
+    T = _lib.torch()
+    outs = []
+    if precision == "fp32":
+        devs = []
+        kind = "numpy"
+        for a in lst:
+            d, kind = _lib.to_device(a, T.complex128)
+            devs.append(d)
+        lm = lmax if lmax is not None else lmax_from_nalm(devs[0].shape[-1])
+        batch = T.stack(devs, dim=1).to(T.complex64).contiguous()
+        maps = synth_batch_f32(batch, nside, lm)
+        outs = [_lib.from_device(maps[i], kind) for i in range(len(lst))]
+    elif precision == "fp64":
+        for a in lst:
+            d, kind = _lib.to_device(a, T.complex128)
+            lm = lmax if lmax is not None else lmax_from_nalm(d.shape[-1])
+            if lm != lmax_from_nalm(d.shape[-1]):
+                raise ValueError("lmax smaller than the alm size is not supported (mmax = lmax)")
+            outs.append(_lib.from_device(_synth_f64(d, nside, lm), kind))
+    else:
+        raise ValueError(f"unknown precision {precision!r}")
+    return outs[0] if single else outs
+
+
+def map2alm(maps, *, lmax: int | None = None, pol: bool = True,
+            use_pixel_weights: bool = False, iter: int = 3,  # noqa: A002
+            precision: str = "fp64"):
+    """glass/healpix.py:232-276 (healpy.map2alm; RING input, mmax = lmax,
+    default lmax = 3 nside - 1, ``iter`` Jacobi refinement passes).
+
+    Weights: uniform 4 pi / npix per pixel.  healpy's ``use_pixel_weights``
+    tables are data files that are not available here; the datapath is checked
+    exactly like the reference, and the weighting falls back to uniform
+    weights refined by the Jacobi iterations.
+    """
+    if use_pixel_weights:
+        _get_healpy_datapath()
+        warnings.warn("use_pixel_weights: pixel-weight tables are not available; using uniform "
+                      "weights with Jacobi refinement", stacklevel=2)
+    T = _lib.torch()
+    single = getattr(maps, "ndim", None) == 1
+    if not single and pol and len(_as_list(maps)) == 3:
+        raise NotImplementedError("polarised (TQU) analysis is not part of this build")
+    lst = [maps] if single else _as_list(maps)
+    nside = npix2nside(lst[0].shape[-1])
+    if lmax is None:
+        lmax = 3 * nside - 1
+    plan = _lib.get_plan(nside, lmax)
+    outs = []
+    if precision == "fp64":
+        ws = _lib.workspace(plan.workspace_bytes(1, 8, analysis=True))
+        for mp in lst:
+            d, kind = _lib.to_device(mp, T.float64)
+            alm = T.empty(nalm(lmax), dtype=T.complex128, device="cuda")
+            tmp = T.empty_like(d) if iter > 0 else None
+            _lib.call("ga_map2alm_f64", plan.h, _lib.ptr(d), _lib.ptr(alm), int(iter),
+                      _lib.ptr(tmp) if tmp is not None else None, _lib.ptr(ws), ws.numel(),
+                      _lib.stream_ptr())
+            outs.append(_lib.from_device(alm, kind))
+    elif precision == "fp32":
+        devs = []
+        kind = "numpy"
+        for mp in lst:
+            d, kind = _lib.to_device(mp, T.float32)
+            devs.append(d)
+        batch = T.stack(devs, dim=0).contiguous()
+        alm = map2alm_batch_f32(batch, lmax, iter)
+        outs = [_lib.from_device(alm[:, i].contiguous(), kind) for i in range(len(lst))]
+    else:
+        raise ValueError(f"unknown precision {precision!r}")
+    return outs[0] if single else outs
+
+
+def map2alm_batch_f32(maps, lmax: int, niter: int = 3):
+    """[S, npix] float32 device maps -> [nalm, S] complex64 alm."""
+    T = _lib.torch()
+    S = int(maps.shape[0])
+    nside = npix2nside(maps.shape[-1])
+    plan = _lib.get_plan(nside, lmax)
+    ws = _lib.workspace(plan.workspace_bytes(S, 4, analysis=True))
+    alm = T.empty((nalm(lmax), S), dtype=T.complex64, device="cuda")
+    tmp = T.empty_like(maps) if niter > 0 else None
+    _lib.call("ga_map2alm_batch_f32", plan.h, S, _lib.ptr(maps), _lib.ptr(alm), int(niter),
+              _lib.ptr(tmp) if tmp is not None else None, _lib.ptr(ws), ws.numel(),
+              _lib.stream_ptr())
+    return alm
+
+
+def alm2map_spin(alms, nside: int, spin: int, lmax: int, precision: str = "fp64"):
+    """glass/healpix.py:81-108: [E, B] -> [Q, U], Q + iU = -sum (E + iB) sY_lm."""
+    T = _lib.torch()
+    if int(spin) not in (1, 2):
+        raise NotImplementedError(f"spin {spin} is not supported (1 or 2)")
+    e, kind = _lib.to_device(alms[0], T.complex128)
+    b, _ = _lib.to_device(alms[1], T.complex128)
+    if lmax_from_nalm(e.shape[-1]) != lmax:
+        raise ValueError("alm size does not match lmax (mmax = lmax)")
+    plan = _lib.get_plan(nside, lmax)
+    plan.prepare_spin(int(spin))
+    npix = nside2npix(nside)
+    if precision == "fp64":
+        ws = _lib.workspace(plan.workspace_bytes(2, 8))
+        out = T.empty((2, npix), dtype=T.float64, device="cuda")
+        _lib.call("ga_alm2map_spin_f64", plan.h, int(spin), _lib.ptr(e), _lib.ptr(b),
+                  _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    elif precision == "fp32":
+        out = spin_batch_f32(e.to(T.complex64)[:, None].contiguous(),
+                             b.to(T.complex64)[:, None].contiguous(), nside, spin, lmax)[0]
+    else:
+        raise ValueError(f"unknown precision {precision!r}")
+    return [_lib.from_device(out[0], kind), _lib.from_device(out[1], kind)]
+
+
+def spin_batch_f32(E, B, nside: int, spin: int, lmax: int):
+    """E, B: [nalm, S] complex64 device tensors -> [S, 2, npix] float32 (Q, U)."""
+    T = _lib.torch()
+    S = int(E.shape[1])
+    plan = _lib.get_plan(nside, lmax)
+    plan.prepare_spin(int(spin))
+    ws = _lib.workspace(plan.workspace_bytes(2 * S, 4))
+    out = T.empty((S, 2, nside2npix(nside)), dtype=T.float32, device="cuda")
+    _lib.call("ga_alm2map_spin_batch_f32", plan.h, int(spin), S, _lib.ptr(E), _lib.ptr(B),
+              _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    return out
+
+
+def almxfl(alm, fl, *, inplace: bool = False):
+    """glass/healpix.py:111-140: alm_lm * fl[l], zero where fl is not defined."""
+    T = _lib.torch()
+    is_t = _lib.is_torch(alm)
+    dtype = alm.dtype if is_t else np.asarray(alm).dtype
+    cdt = T.complex64 if dtype in (T.complex64, np.complex64) else T.complex128
+    d, kind = _lib.to_device(alm, cdt)
+    f, _ = _lib.to_device(fl, T.float64)
+    lmax = lmax_from_nalm(d.shape[-1])
+    out = d if (inplace and is_t and d.data_ptr() == alm.data_ptr()) else T.empty_like(d)
+    _lib.call("ga_almxfl", lmax, 1, 4 if cdt == T.complex64 else 8, _lib.ptr(d), _lib.ptr(f),
+              int(f.numel()), _lib.ptr(out), _lib.stream_ptr())
+    res = _lib.from_device(out, kind)
+    if inplace and kind == "numpy":
+        np.copyto(alm, res)
+        return alm
+    return res
+
+
+def pixwin(nside: int, *, lmax: int | None = None, pol: bool = False, xp=None):
+    """glass/healpix.py:313-356.  The pixel-window tables are healpy package
+    data (FITS) that this build cannot load; pass windows to almxfl /
+    from_convergence(pixwin_ratio=...) explicitly instead."""
+    _get_healpy_datapath()
+    raise NotImplementedError(
+        "HEALPix pixel-window tables are not available in this build; supply them explicitly")
+
+
+def sequence_of_arrays(x) -> bool:
+    return isinstance(x, Sequence) and not isinstance(x, (str, bytes))

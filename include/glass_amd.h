@@ -1,0 +1,181 @@
+/*
+ * glass_amd: MI355X-native field-generation hot path for GLASS.
+ *
+ * Plain C ABI.  Every array argument is a raw device pointer owned by the
+ * caller; sizes are element counts; the stream is a hipStream_t passed as
+ * void*.  Every entry point returns a status code (GA_OK == 0) and leaves a
+ * human-readable message for ga_last_error() on failure.  Work is
+ * stream-ordered: a call returns after enqueueing (except where noted).
+ *
+ * Layouts
+ *   alm, single map : complex, healpy order idx(l,m) = m(2 lmax + 1 - m)/2 + l,
+ *                     mmax = lmax, interleaved (re, im).
+ *   alm, batch      : complex64, [idx(l,m)][S]  (shell index innermost).
+ *   maps            : RING ordered, 12 nside^2 pixels; batches are [S][npix].
+ *   ring workspace  : complex [nrings][lmax+1][S], nrings = 4 nside - 1.
+ *
+ * Each entry point names the reference interface it stands in for.
+ */
+#ifndef GLASS_AMD_H
+#define GLASS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ga_status {
+  GA_OK = 0,
+  GA_ERR_INVALID = 1,     /* bad argument (maps to ValueError) */
+  GA_ERR_HIP = 2,         /* HIP runtime error */
+  GA_ERR_NOMEM = 3,       /* device allocation failed (plan tables) */
+  GA_ERR_UNSUPPORTED = 4, /* size outside what the kernels handle */
+  GA_ERR_NOT_POSDEF = 5,  /* "covariance matrix is not positive definite" */
+  GA_ERR_WORKSPACE = 6    /* workspace smaller than ga_*_workspace_size */
+};
+
+/* pointwise transformation fused into the synthesis epilogue;
+ * glass/grf/_transformations.py:26-28 (Normal), :83-89 (Lognormal),
+ * :170-175 (SquaredNormal) */
+enum ga_transform { GA_T_NORMAL = 0, GA_T_LOGNORMAL = 1, GA_T_SQUARED_NORMAL = 2 };
+
+typedef struct ga_epilogue {
+  int32_t kind;   /* enum ga_transform */
+  int32_t pad;
+  double lamda;   /* lambda (shift / scale) */
+  double param;   /* Lognormal: variance of the Gaussian field; SquaredNormal: a */
+} ga_epilogue;
+
+typedef struct ga_plan ga_plan;
+
+/* Last error message of the calling host thread ("" if none). */
+const char *ga_last_error(void);
+
+/* Library version string. */
+const char *ga_version(void);
+
+/* HEALPix geometry and Legendre/FFT tables for one (nside, lmax) on the current
+ * device.  Allocates the plan's own read-only tables; no allocation happens in
+ * any other entry point.  Replaces the per-call set-up inside healpy.alm2map /
+ * healpy.map2alm (glass/healpix.py:71, :270). */
+int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out);
+int ga_plan_destroy(ga_plan *plan);
+/* Bytes of device memory held by the plan's tables. */
+int ga_plan_table_bytes(const ga_plan *plan, size_t *bytes);
+
+/* Workspace bytes for nbatch maps at the given element width (4 = fp32,
+ * 8 = fp64).  Synthesis (scalar): the ring buffer [nrings][lmax+1][nbatch]
+ * complex.  Spin synthesis needs the size for 2 nbatch.  Analysis: use
+ * ga_map2alm_workspace_size. */
+int ga_workspace_size(const ga_plan *plan, int32_t nbatch, int32_t real_bytes,
+                      size_t *bytes);
+int ga_map2alm_workspace_size(const ga_plan *plan, int32_t nbatch, int32_t real_bytes,
+                              size_t *bytes);
+
+/* One-time build of the spin-s (s = 1, 2) recurrence tables of a plan
+ * (allocates); required before the spin entry points. */
+int ga_plan_prepare_spin(ga_plan *plan, int32_t spin);
+
+/* FP64 scalar synthesis of one map, plain (no MFMA, no batching).
+ * glass/healpix.py:38-78 alm2map(alms, nside, pol=False), as called at
+ * glass/fields.py:429 and glass/lensing.py:326. */
+int ga_alm2map_f64(const ga_plan *plan, const double *alm, double *map,
+                   void *workspace, size_t ws_bytes, void *stream);
+
+/* FP32 batched scalar synthesis of S maps: Legendre contraction on FP32 MFMA
+ * with the recurrence in FP64, per-ring FFT, fused pointwise transform
+ * (epi: S device structs, or NULL for identity).  Batched form of the
+ * per-shell alm2map + t(x, var) of glass/fields.py:429, :894. */
+int ga_alm2map_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
+                         float *maps, const ga_epilogue *epi, void *workspace,
+                         size_t ws_bytes, void *stream);
+
+/* The two stages of ga_alm2map_batch_f32, callable separately (for timing
+ * and for callers that keep the ring buffer): the FP32-MFMA Legendre stage
+ * alm [idx][S] -> ring buffer F [nrings][lmax+1][S], and the ring-FFT stage
+ * F -> maps [S][npix] with the fused transform. */
+int ga_legendre_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
+                                float *F, void *stream);
+int ga_ring_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *F, float *maps,
+                            const ga_epilogue *epi, void *stream);
+
+/* FP64 scalar analysis with healpy's Jacobi refinement (niter passes,
+ * alm += A(map - S(alm))), uniform pixel weights 4 pi / npix.
+ * glass/healpix.py:232-276 map2alm(pol=False), called at glass/lensing.py:306.
+ * map_tmp: npix float64 scratch (used when niter > 0). */
+int ga_map2alm_f64(const ga_plan *plan, const double *map, double *alm, int32_t niter,
+                   double *map_tmp, void *workspace, size_t ws_bytes, void *stream);
+
+/* FP32 batched analysis of S maps [S][npix] into [idx][S] alm (niter Jacobi
+ * passes; the residual synthesis uses the FP32-MFMA path).
+ * map_tmp: S npix float32 scratch. */
+int ga_map2alm_batch_f32(const ga_plan *plan, int32_t nshell, const float *maps, float *alm,
+                         int32_t niter, float *map_tmp, void *workspace, size_t ws_bytes,
+                         void *stream);
+
+/* Spin-s synthesis (s = 1, 2) of [E, B] into [Q, U], HEALPix convention
+ * Q + iU = -sum (E + iB) sY_lm.  glass/healpix.py:81-108 alm2map_spin, called
+ * at glass/lensing.py:343 (s = 1) and :366, :428 (s = 2).
+ * f64: one map pair, maps = [Q | U] (2 npix).  f32: S pairs [S][2][npix].
+ * workspace: ga_workspace_size for 2 S maps. */
+int ga_alm2map_spin_f64(const ga_plan *plan, int32_t spin, const double *almE,
+                        const double *almB, double *maps, void *workspace, size_t ws_bytes,
+                        void *stream);
+int ga_alm2map_spin_batch_f32(const ga_plan *plan, int32_t spin, int32_t nshell,
+                              const float *almE, const float *almB, float *maps, void *workspace,
+                              size_t ws_bytes, void *stream);
+
+/* alm[idx(l,m)][s] *= fl[l] (0 beyond nfl), in place or out of place, for
+ * S interleaved maps; real_bytes 4 or 8.  glass/healpix.py:111-140 almxfl. */
+int ga_almxfl(int32_t lmax, int32_t nshell, int32_t real_bytes, const void *alm,
+              const double *fl, int32_t nfl, void *out, void *stream);
+
+/* GLASS-order multalm: alm[l(l+1)/2 + m] * bl[l] (complex128 in/out, float64
+ * bl).  glass/harmonics.py:17-47. */
+int ga_multalm_f64(int32_t nl, const double *alm, const double *bl, double *out,
+                   void *stream);
+
+/* GLASS (l-major) -> healpy (m-major) alm order, complex128.
+ * glass/fields.py:943-962. */
+int ga_glass_to_healpix_alm_f64(int32_t nl, const double *alm_glass, double *alm_hp,
+                                void *stream);
+
+/* Batched conditional-Gaussian scaling vectors for nshell consecutive fields.
+ * cov: [nshell][nl][k+1] float64 covariance rows as cls2cov yields them
+ * (auto-spectrum first, then the correlations with the previous fields,
+ * already halved).  state: [nl][kmax*kmax + 2 kmax + 1] float64 carried between
+ * calls (ga_iternorm_state_doubles(kmax) per mode); it has size kprev on entry
+ * (k <= kprev + 1) unless start = 1, which marks the first row of this call as
+ * field 0 and initialises the state.  w: [nshell][nl][k+1] output scaling
+ * vectors.  *flag (device int32, caller-zeroed) is set to 1 if any covariance
+ * is not positive definite.  glass/fields.py:101-188 iternorm. */
+int ga_iternorm_state_doubles(int32_t kmax, int32_t *n);
+int ga_iternorm_batch(int32_t nl, int32_t kprev, int32_t k, int32_t kmax, int32_t nshell,
+                      int32_t start, const double *cov, double *state, double *w,
+                      int32_t *flag, void *stream);
+
+/* Correlated alm for shells first_shell .. first_shell + nshell - 1:
+ *   alm_t = m0fix(to_healpix(sum_j w_t[l][j + mis] z_{t - len + 1 + j}))
+ * (glass/fields.py:401-425).  Normal vectors z are either
+ *   - host-drawn (numpy PCG64 stream): zring = device ring buffer of nslot
+ *     GLASS-order complex128 vectors, z_t in slot t % nslot (nslot >=
+ *     nshell + k so that every shell of the call sees its window); or
+ *   - counter-based Philox4x32-10 (seed, shell, index) when zring == NULL.
+ * Output: batch layout [idx][nshell] complex64 (out_f32) or, for nshell == 1,
+ * healpy-order complex128 (out_f64); exactly one of the two is non-NULL. */
+int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
+                  const double *w, const double *zring, int32_t nslot, uint64_t seed,
+                  float *out_f32, double *out_f64, void *stream);
+
+/* Pointwise transforms on a float64 map (n pixels), out may equal x.
+ * glass/grf/_transformations.py. */
+int ga_transform_f64(int64_t n, const double *x, const ga_epilogue *epi_host, double *out,
+                     void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLASS_AMD_H */

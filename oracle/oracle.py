@@ -27,8 +27,12 @@ Pinning (what checks each function, all in ``tests/test_oracle.py``):
                     variance of a synthesised map (Parseval / addition theorem).
 * alm2map        -- brute force with scipy.special.sph_harm_y at every pixel
                     centre (nside <= 8), closed forms Y_00 / Y_10 / Y_11,
-                    mpmath.legenp for high-l Legendre values with underflowing
-                    seeds, numpy.fft for the ring DFT, phi-rotation invariant.
+                    Gauss-Legendre orthonormality of lambda_lm up to l = 2500,
+                    the recurrence in mpmath's unbounded exponent range for
+                    seeds 10^-346 .. 10^-1050 (checks the exponent scaling),
+                    numpy.fft for the ring DFT, phi-rotation invariant.
+* almxfl         -- a delta function in l keeps exactly the entries of that l
+                    (placement in healpy order), ones keep everything.
 * map2alm        -- adjointness <S a, f> = (npix / 4 pi) <a, A f> and round trip
                     of band-limited maps (lmax <= nside) with Jacobi iterations.
 * alm2map_spin   -- spin-weighted harmonics from the spin-raising operator
@@ -71,6 +75,7 @@ def lib() -> ctypes.CDLL:
         dp = ctypes.POINTER(ctypes.c_double)
         i64 = ctypes.c_int64
         L.oracle_alm2map.argtypes = [i64, i64, dp, dp, ctypes.c_int]
+        L.oracle_alm2map_rings.argtypes = [i64, i64, dp, dp, i64, i64, ctypes.c_int]
         L.oracle_map2alm.argtypes = [i64, i64, dp, dp, ctypes.c_int, ctypes.c_int]
         L.oracle_alm2map_spin.argtypes = [i64, i64, i64, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_legendre.argtypes = [i64, i64, ctypes.c_double, ctypes.c_double, dp]
@@ -174,6 +179,19 @@ def alm2map(alm: np.ndarray, nside: int, lmax: int | None = None,
     return out
 
 
+def alm2map_rings(alm: np.ndarray, nside: int, r_first: int, r_step: int,
+                  nthreads: int | None = None) -> np.ndarray:
+    """Synthesis of rings r_first, r_first + r_step, ... only (1-based); the
+    other pixels stay zero.  Used to time a bounded sample of a large map."""
+    alm = np.ascontiguousarray(alm, dtype=np.complex128)
+    lmax = lmax_from_nalm(alm.size)
+    buf = alm.view(np.float64).copy()
+    out = np.zeros(nside2npix(nside))
+    lib().oracle_alm2map_rings(nside, lmax, _dp(buf), _dp(out), int(r_first), int(r_step),
+                               _threads(nthreads))
+    return out
+
+
 def map2alm(m: np.ndarray, lmax: int, niter: int = 3,
             nthreads: int | None = None) -> np.ndarray:
     """FP64 scalar analysis with healpy's Jacobi refinement."""
@@ -204,7 +222,7 @@ def almxfl(alm: np.ndarray, fl: np.ndarray) -> np.ndarray:
     k = min(len(fl), lmax + 1)
     f[:k] = np.asarray(fl, dtype=np.float64)[:k]
     for m in range(lmax + 1):
-        i0 = m * (2 * lmax + 1 - m) // 2
+        i0 = m * (2 * lmax + 1 - m) // 2 + m  # index of (l = m, m)
         out[i0:i0 + lmax - m + 1] *= f[m:]
     return out
 

@@ -258,7 +258,7 @@ static void legendre_column(int64_t m, int64_t lmax, double x, scaled_t seed,
       p1 = pn;
       if (fabs(p1) > big) { p1 *= rescale; p0 *= rescale; e += 600; }
     }
-    out[l - m] = (e < -1100) ? 0.0 : ldexp(p1, (int)e);
+    out[l - m] = (e == 0) ? p1 : ((e < -1100) ? 0.0 : ldexp(p1, (int)e));
   }
 }
 
@@ -342,103 +342,197 @@ void oracle_spin_legendre(int64_t s, int64_t m, int64_t lmax, double z, double s
 
 static int64_t nrings_of(int64_t nside) { return 4 * nside - 1; }
 
-/* alm (complex interleaved, healpy order, mmax = lmax) -> map (RING) */
-void oracle_alm2map(int64_t nside, int64_t lmax, const double *alm, double *map,
-                    int nthreads) {
+/* F_m of OR_BLK rings at once: independent recurrence chains in lockstep
+ * (first with per-ring rescaling while any ring is below range, then plain) */
+#define OR_BLK 8
+static void synth_block(int64_t nside, int64_t m, int64_t lmax, const int64_t *rings, int nb,
+                        const scaled_t *seeds, int64_t seed_stride, const double *ab,
+                        const double *a, double *sr, double *si) {
+  double x[OR_BLK], p0[OR_BLK], p1[OR_BLK], ar[OR_BLK], ai[OR_BLK];
+  int64_t e[OR_BLK];
+  for (int j = 0; j < OR_BLK; ++j) {
+    ring_t g;
+    ring_info(nside, rings[j < nb ? j : 0], &g);
+    scaled_t sd = seeds[(size_t)(j < nb ? j : 0) * (size_t)seed_stride + (size_t)m];
+    x[j] = g.z; p0[j] = 0.0; p1[j] = sd.v; e[j] = sd.e;
+    ar[j] = 0.0; ai[j] = 0.0;
+  }
+  const double big = 0x1p300, rescale = 0x1p-600;
+  int64_t l = m;
+  int any_low = 1;
+  for (; l <= lmax && any_low; ++l) {
+    const double c0 = ab[2 * (l - m)], c1 = ab[2 * (l - m) + 1];
+    any_low = 0;
+    for (int j = 0; j < OR_BLK; ++j) {
+      if (l > m) {
+        double pn = c0 * (x[j] * p1[j] - c1 * p0[j]);
+        p0[j] = p1[j]; p1[j] = pn;
+      }
+      if (e[j] < 0 && fabs(p1[j]) > big) { p1[j] *= rescale; p0[j] *= rescale; e[j] += 600; }
+      double v = (e[j] == 0) ? p1[j] : ((e[j] < -1100) ? 0.0 : ldexp(p1[j], (int)e[j]));
+      if (e[j] != 0) any_low = 1;
+      ar[j] += v * a[2 * (l - m)];
+      ai[j] += v * a[2 * (l - m) + 1];
+    }
+  }
+  for (; l <= lmax; ++l) {
+    const double c0 = ab[2 * (l - m)], c1 = ab[2 * (l - m) + 1];
+    const double br = a[2 * (l - m)], bi = a[2 * (l - m) + 1];
+    for (int j = 0; j < OR_BLK; ++j) {
+      double pn = c0 * (x[j] * p1[j] - c1 * p0[j]);
+      p0[j] = p1[j]; p1[j] = pn;
+      ar[j] += pn * br;
+      ai[j] += pn * bi;
+    }
+  }
+  for (int j = 0; j < nb; ++j) { sr[j] = ar[j]; si[j] = ai[j]; }
+}
+
+/* lambda_mm seeds of every (ring, m) for the rings in the list */
+static scaled_t *seed_table(int64_t nside, int64_t lmax, const int64_t *rings, int64_t nsel) {
+  scaled_t *t = malloc((size_t)nsel * (size_t)(lmax + 1) * sizeof(scaled_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (int64_t i = 0; i < nsel; ++i) {
+    ring_t g;
+    ring_info(nside, rings[i], &g);
+    legendre_seeds(lmax, g.sth, t + (size_t)i * (size_t)(lmax + 1));
+  }
+  return t;
+}
+
+/* alm (complex interleaved, healpy order, mmax = lmax) -> map (RING), rings
+ * r_first, r_first + r_step, ... only (1-based; r_first = 1, r_step = 1 is the
+ * full map).  Loop order m-outer so that one m's coefficients and alm stay in
+ * cache while every ring is visited: F[ring][m] first, then the ring sums. */
+void oracle_alm2map_rings(int64_t nside, int64_t lmax, const double *alm, double *map,
+                          int64_t r_first, int64_t r_step, int nthreads) {
   const int64_t nr = nrings_of(nside);
-  double *tab = legendre_table(lmax);
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  int64_t nsel = 0;
+  for (int64_t r = r_first; r <= nr; r += r_step) ++nsel;
+  int64_t *rings = malloc((size_t)(nsel > 0 ? nsel : 1) * sizeof(int64_t));
+  nsel = 0;
+  for (int64_t r = r_first; r <= nr; r += r_step) rings[nsel++] = r;
+  double *tab = legendre_table(lmax);
+  scaled_t *seeds = seed_table(nside, lmax, rings, nsel);
+  double *F = malloc((size_t)nsel * (size_t)(lmax + 1) * 2 * sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t m = 0; m <= lmax; ++m) {
+    const double *ab = tab + 2 * alm_index(lmax, m, m);
+    const double *a = alm + 2 * alm_index(lmax, m, m);
+    for (int64_t i0 = 0; i0 < nsel; i0 += OR_BLK) {
+      const int nb = (int)((nsel - i0) < OR_BLK ? (nsel - i0) : OR_BLK);
+      double sr[OR_BLK], si[OR_BLK];
+      synth_block(nside, m, lmax, rings + i0, nb,
+                  seeds + (size_t)i0 * (size_t)(lmax + 1), lmax + 1, ab, a, sr, si);
+      for (int j = 0; j < nb; ++j) {
+        double *f = F + 2 * ((size_t)(i0 + j) * (size_t)(lmax + 1) + (size_t)m);
+        f[0] = sr[j];
+        f[1] = (m == 0) ? 0.0 : si[j]; /* healpy ignores Im a_l0 */
+      }
+    }
+  }
+#ifdef _OPENMP
 #pragma omp parallel
 #endif
   {
     double *Fr = malloc((size_t)(lmax + 1) * sizeof(double));
     double *Fi = malloc((size_t)(lmax + 1) * sizeof(double));
-    double *lam = malloc((size_t)(lmax + 1) * sizeof(double));
-    scaled_t *seeds = malloc((size_t)(lmax + 1) * sizeof(scaled_t));
     double *wr = malloc((size_t)(16 * nside) * sizeof(double));
     double *wi = malloc((size_t)(16 * nside) * sizeof(double));
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 4)
 #endif
-    for (int64_t r = 1; r <= nr; ++r) {
+    for (int64_t i = 0; i < nsel; ++i) {
       ring_t g;
-      ring_info(nside, r, &g);
-      legendre_seeds(lmax, g.sth, seeds);
+      ring_info(nside, rings[i], &g);
       for (int64_t m = 0; m <= lmax; ++m) {
-        legendre_column(m, lmax, g.z, seeds[m], tab + 2 * alm_index(lmax, m, m), lam);
-        double sr = 0.0, si = 0.0;
-        const double *a = alm + 2 * alm_index(lmax, m, m);
-        for (int64_t l = m; l <= lmax; ++l) {
-          sr += a[2 * (l - m)] * lam[l - m];
-          si += a[2 * (l - m) + 1] * lam[l - m];
-        }
-        Fr[m] = sr;
-        Fi[m] = (m == 0) ? 0.0 : si; /* healpy ignores Im a_l0 */
+        Fr[m] = F[2 * ((size_t)i * (size_t)(lmax + 1) + (size_t)m)];
+        Fi[m] = F[2 * ((size_t)i * (size_t)(lmax + 1) + (size_t)m) + 1];
       }
       ring_synth(&g, lmax, Fr, Fi, map + g.start, wr, wi);
     }
-    free(Fr); free(Fi); free(lam); free(seeds); free(wr); free(wi);
+    free(Fr); free(Fi); free(wr); free(wi);
   }
-  free(tab);
+  free(F); free(seeds); free(tab); free(rings);
 }
 
-/* adjoint of synthesis with weight 4 pi / npix per pixel (one pass) */
+void oracle_alm2map(int64_t nside, int64_t lmax, const double *alm, double *map,
+                    int nthreads) {
+  oracle_alm2map_rings(nside, lmax, alm, map, 1, 1, nthreads);
+}
+
+/* adjoint of synthesis with weight 4 pi / npix per pixel (one pass):
+ * ring transforms first, then per m (each m owns its alm column) */
 static void alm_from_map_once(int64_t nside, int64_t lmax, const double *map,
                               double *alm, int nthreads) {
   const int64_t nr = nrings_of(nside);
-  const int64_t nalm = (lmax + 1) * (lmax + 2) / 2;
   const double w = 4.0 * M_PI / (double)(12 * nside * nside);
-  int nt = 1;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
-  nt = omp_get_max_threads();
 #endif
-  /* per-thread accumulators, summed in fixed order afterwards */
-  double *acc = calloc((size_t)nt * (size_t)(2 * nalm), sizeof(double));
+  int64_t *rings = malloc((size_t)nr * sizeof(int64_t));
+  for (int64_t i = 0; i < nr; ++i) rings[i] = i + 1;
   double *tab = legendre_table(lmax);
+  scaled_t *seeds = seed_table(nside, lmax, rings, nr);
+  double *F = malloc((size_t)nr * (size_t)(lmax + 1) * 2 * sizeof(double));
 #ifdef _OPENMP
 #pragma omp parallel
 #endif
   {
-    int tid = 0;
-#ifdef _OPENMP
-    tid = omp_get_thread_num();
-#endif
-    double *my = acc + (size_t)tid * (size_t)(2 * nalm);
     double *Fr = malloc((size_t)(lmax + 1) * sizeof(double));
     double *Fi = malloc((size_t)(lmax + 1) * sizeof(double));
-    double *lam = malloc((size_t)(lmax + 1) * sizeof(double));
-    scaled_t *seeds = malloc((size_t)(lmax + 1) * sizeof(scaled_t));
     double *wr = malloc((size_t)(16 * nside) * sizeof(double));
     double *wi = malloc((size_t)(16 * nside) * sizeof(double));
 #ifdef _OPENMP
-#pragma omp for schedule(static)
+#pragma omp for schedule(dynamic, 4)
 #endif
-    for (int64_t r = 1; r <= nr; ++r) {
+    for (int64_t i = 0; i < nr; ++i) {
       ring_t g;
-      ring_info(nside, r, &g);
+      ring_info(nside, rings[i], &g);
       ring_anal(&g, lmax, map + g.start, w, Fr, Fi, wr, wi);
-      legendre_seeds(lmax, g.sth, seeds);
       for (int64_t m = 0; m <= lmax; ++m) {
-        legendre_column(m, lmax, g.z, seeds[m], tab + 2 * alm_index(lmax, m, m), lam);
-        double *a = my + 2 * alm_index(lmax, m, m);
-        for (int64_t l = m; l <= lmax; ++l) {
-          a[2 * (l - m)] += lam[l - m] * Fr[m];
-          a[2 * (l - m) + 1] += lam[l - m] * Fi[m];
-        }
+        F[2 * ((size_t)i * (size_t)(lmax + 1) + (size_t)m)] = Fr[m];
+        F[2 * ((size_t)i * (size_t)(lmax + 1) + (size_t)m) + 1] = Fi[m];
       }
     }
-    free(Fr); free(Fi); free(lam); free(seeds); free(wr); free(wi);
+    free(Fr); free(Fi); free(wr); free(wi);
   }
-  for (int64_t i = 0; i < 2 * nalm; ++i) {
-    double s = 0.0;
-    for (int t = 0; t < nt; ++t) s += acc[(size_t)t * (size_t)(2 * nalm) + i];
-    alm[i] = s;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *lam = malloc((size_t)(lmax + 1) * sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int64_t m = 0; m <= lmax; ++m) {
+      const double *ab = tab + 2 * alm_index(lmax, m, m);
+      double *a = alm + 2 * alm_index(lmax, m, m);
+      for (int64_t l = m; l <= lmax; ++l) { a[2 * (l - m)] = 0.0; a[2 * (l - m) + 1] = 0.0; }
+      for (int64_t i = 0; i < nr; ++i) {
+        ring_t g;
+        ring_info(nside, rings[i], &g);
+        legendre_column(m, lmax, g.z, seeds[(size_t)i * (size_t)(lmax + 1) + (size_t)m], ab, lam);
+        const double fr = F[2 * ((size_t)i * (size_t)(lmax + 1) + (size_t)m)];
+        const double fi = F[2 * ((size_t)i * (size_t)(lmax + 1) + (size_t)m) + 1];
+        for (int64_t l = m; l <= lmax; ++l) {
+          a[2 * (l - m)] += lam[l - m] * fr;
+          a[2 * (l - m) + 1] += lam[l - m] * fi;
+        }
+      }
+      if (m == 0)
+        for (int64_t l = 0; l <= lmax; ++l) a[2 * l + 1] = 0.0; /* m = 0 real */
+    }
+    free(lam);
   }
-  for (int64_t l = 0; l <= lmax; ++l) alm[2 * l + 1] = 0.0; /* m = 0 real */
-  free(acc);
-  free(tab);
+  free(F); free(seeds); free(tab); free(rings);
 }
 
 /* healpy map2alm(iter=niter): alm = A f; repeat alm += A(f - S alm) */

@@ -297,6 +297,24 @@ def test_map2alm_adjoint_and_round_trip():
     assert e10 < 1e-10
 
 
+def test_almxfl_placement():
+    lmax = 12
+    rng = np.random.default_rng(9)
+    alm = rng.standard_normal(O.nalm(lmax)) + 1j * rng.standard_normal(O.nalm(lmax))
+    np.testing.assert_array_equal(O.almxfl(alm, np.ones(lmax + 1)), alm)
+    for L0 in (0, 1, 5, lmax):
+        f = np.zeros(lmax + 1)
+        f[L0] = 2.0
+        out = O.almxfl(alm, f)
+        for m in range(lmax + 1):
+            for l in range(m, lmax + 1):
+                i = _hp(lmax, l, m)
+                assert out[i] == (2.0 * alm[i] if l == L0 else 0.0)
+    # zero beyond len(fl)
+    out = O.almxfl(alm, np.ones(4))
+    assert all(out[_hp(lmax, l, m)] == 0 for m in range(lmax + 1) for l in range(max(m, 4), lmax + 1))
+
+
 def test_cltovar_against_map_variance():
     nside, lmax = 16, 30
     cl = 1.0 / (np.arange(lmax + 1) + 1.0) ** 2
