@@ -10,13 +10,15 @@ library or GPU raises immediately.
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 import threading
 
 import numpy as np
 
 _HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = _HERE / "_lib" / "libglass_amd.so"
+# GLASS_AMD_LIBRARY selects an alternate build of the same library (A/B timing)
+LIB_PATH = pathlib.Path(os.environ.get("GLASS_AMD_LIBRARY", _HERE / "_lib" / "libglass_amd.so"))
 
 GA_OK = 0
 GA_ERR_INVALID = 1
@@ -25,6 +27,9 @@ GA_ERR_NOMEM = 3
 GA_ERR_UNSUPPORTED = 4
 GA_ERR_NOT_POSDEF = 5
 GA_ERR_WORKSPACE = 6
+
+TUNE_MFMA_TILES = 1
+TUNE_MFMA_CHUNK = 2
 
 T_NORMAL = 0
 T_LOGNORMAL = 1
@@ -44,6 +49,8 @@ _sz = ctypes.c_size_t
 
 # name -> (argtypes); every entry point returns int status
 _SIGNATURES = {
+    "ga_set_tuning": [_i32, _i32],
+    "ga_get_tuning": [_i32, ctypes.POINTER(_i32)],
     "ga_plan_create": [_i32, _i32, ctypes.POINTER(_vp)],
     "ga_plan_destroy": [_vp],
     "ga_plan_table_bytes": [_vp, ctypes.POINTER(_sz)],

@@ -13,8 +13,12 @@ __global__ void residual_kernel(int64_t n, const T *__restrict__ a, T *__restric
   if (i < n) b[i] = a[i] - b[i];
 }
 
+// same size as ga_workspace_size: fp32 ring buffers hold whole groups of
+// kMfmaGroup shells
 size_t ring_bytes(const ga_plan *p, int nbatch, int rb) {
-  return size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nbatch) * 2 * size_t(rb);
+  size_t b = 0;
+  (void)ga_workspace_size(p, nbatch, rb, &b);
+  return b;
 }
 size_t partial_bytes(const ga_plan *p, int nbatch, int rb) {
   return size_t(anal_partial_slots(p)) * size_t(p->nalm) * size_t(nbatch) * 2 * size_t(rb);
@@ -55,7 +59,7 @@ extern "C" int ga_alm2map_f64(const ga_plan *p, const double *alm, double *map, 
   double *F = (double *)ws;
   int rc = launch_legendre_synth_f64(p, alm, F, s);
   if (rc) return rc;
-  return launch_ring_synth_f64(p, 1, F, map, nullptr, s);
+  return launch_ring_synth_f64(p, 1, 1, F, map, nullptr, s);
 }
 
 extern "C" int ga_alm2map_batch_f32(const ga_plan *p, int32_t S, const float *alm, float *maps,
@@ -69,7 +73,7 @@ extern "C" int ga_alm2map_batch_f32(const ga_plan *p, int32_t S, const float *al
   float *F = (float *)ws;
   int rc = launch_legendre_synth_mfma(p, S, alm, F, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, S, F, maps, epi, s);
+  return launch_ring_synth_f32(p, S, kMfmaGroup, F, maps, epi, s);
 }
 
 extern "C" int ga_legendre_synth_batch_f32(const ga_plan *p, int32_t S, const float *alm,
@@ -83,7 +87,7 @@ extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float 
                                        const ga_epilogue *epi, void *stream) {
   GA_PLAN_OK(p);
   GA_REQUIRE(S >= 1 && F && maps, GA_ERR_INVALID, "invalid argument");
-  return launch_ring_synth_f32(p, S, F, maps, epi, (hipStream_t)stream);
+  return launch_ring_synth_f32(p, S, kMfmaGroup, F, maps, epi, (hipStream_t)stream);
 }
 
 template <typename T>
@@ -105,11 +109,12 @@ static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter
       if (sizeof(T) == 8) {
         rc = launch_legendre_synth_f64(p, (const double *)alm, (double *)F, s);
         if (rc) return rc;
-        rc = launch_ring_synth_f64(p, 1, (const double *)F, (double *)map_tmp, nullptr, s);
+        rc = launch_ring_synth_f64(p, 1, 1, (const double *)F, (double *)map_tmp, nullptr, s);
       } else {
         rc = launch_legendre_synth_mfma(p, S, (const float *)alm, (float *)F, s);
         if (rc) return rc;
-        rc = launch_ring_synth_f32(p, S, (const float *)F, (float *)map_tmp, nullptr, s);
+        rc = launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp, nullptr,
+                                   s);
       }
       if (rc) return rc;
       hipLaunchKernelGGL(residual_kernel<T>, dim3(unsigned((npx + 255) / 256)), dim3(256), 0, s,
@@ -157,7 +162,7 @@ extern "C" int ga_alm2map_spin_f64(const ga_plan *p, int32_t spin, const double 
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_spin_synth_f64(p, spin, 1, E, B, (double *)ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f64(p, 2, (const double *)ws, maps, nullptr, s);
+  return launch_ring_synth_f64(p, 2, 2, (const double *)ws, maps, nullptr, s);
 }
 
 extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t S,
@@ -170,5 +175,5 @@ extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_spin_synth_f32(p, spin, S, E, B, (float *)ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, 2 * S, (const float *)ws, maps, nullptr, s);
+  return launch_ring_synth_f32(p, 2 * S, 2 * S, (const float *)ws, maps, nullptr, s);
 }

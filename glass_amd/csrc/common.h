@@ -43,6 +43,13 @@ void clear_error();
     }                                                                          \
   } while (0)
 
+// ---- tuning knobs (set through ga_set_tuning; read by the launchers) ----
+struct ga_tuning {
+  int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto)
+  int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
+};
+ga_tuning &tuning();
+
 // ---- sizes ----------------------------------------------------------------
 inline int64_t nalm_of(int64_t lmax) { return (lmax + 1) * (lmax + 2) / 2; }
 inline int64_t npix_of(int64_t nside) { return 12 * nside * nside; }

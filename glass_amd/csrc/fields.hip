@@ -163,6 +163,70 @@ __global__ void fields_alm_kernel(int nl, int k, int first_shell, int nshell, in
   else out64[i] = double2{re, im};
 }
 
+// Device-RNG form, one thread per healpy index walking the shells in order:
+// each normal vector entry z_t[g] is generated once and kept in a sliding
+// register window of k+1 values; the [idx][S] output is staged through LDS in
+// chunks of 16 shells so that stores are contiguous 128-byte rows.
+constexpr int kWinMax = 8;  // k + 1 <= kWinMax
+constexpr int kFaBlock = 256, kFaChunk = 16;
+
+__global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
+    int nl, int k, int first_shell, int nshell, int64_t nalm, const double *__restrict__ w,
+    uint64_t seed, float2 *__restrict__ out32) {
+  __shared__ float2 tile[kFaBlock][kFaChunk + 1];
+  const int tid = threadIdx.x;
+  const int64_t i0 = int64_t(blockIdx.x) * kFaBlock;
+  const int64_t i = i0 + tid;
+  const bool valid = i < nalm;
+  int l = 0, m = 0;
+  if (valid) hp_decode(i, nl - 1, l, m);
+  const int64_t g = int64_t(l) * (l + 1) / 2 + m;
+  // window zz[j] = z_{t - k + j} for the current shell t
+  double2 zz[kWinMax];
+#pragma unroll
+  for (int j = 0; j < kWinMax; ++j) {
+    const int64_t tz = int64_t(first_shell) - k + j;
+    zz[j] = (j <= k && tz >= 0 && valid) ? device_normal_pair(seed, tz, g) : double2{0.0, 0.0};
+  }
+  const int nrows = int(nalm - i0 < kFaBlock ? nalm - i0 : kFaBlock);
+  for (int s0 = 0; s0 < nshell; s0 += kFaChunk) {
+    const int cnt = nshell - s0 < kFaChunk ? nshell - s0 : kFaChunk;
+    for (int c = 0; c < cnt; ++c) {
+      const int s = s0 + c;
+      const int64_t t = int64_t(first_shell) + s;
+      if (s > 0) {
+#pragma unroll
+        for (int j = 0; j + 1 < kWinMax; ++j)
+          if (j < k) zz[j] = zz[j + 1];
+        double2 zn = valid ? device_normal_pair(seed, t, g) : double2{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kWinMax; ++j)
+          if (j == k) zz[j] = zn;
+      }
+      const int len = int(t + 1 < k + 1 ? t + 1 : k + 1);
+      const int mis = k + 1 - len;
+      const double *wl = w + (int64_t(s) * nl + l) * (k + 1);
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int j = 0; j < kWinMax; ++j) {
+        if (j >= mis && j <= k) {
+          re = fma(wl[j], zz[j].x, re);
+          im = fma(wl[j], zz[j].y, im);
+        }
+      }
+      if (m == 0) { re += im; im = 0.0; }
+      tile[tid][c] = float2{float(re), float(im)};
+    }
+    __syncthreads();
+    // cooperative store: element (r, c) of the tile -> out[(i0 + r) nshell + s0 + c]
+    for (int e = tid; e < nrows * cnt; e += kFaBlock) {
+      const int r = e / cnt, c = e - r * cnt;
+      out32[(i0 + r) * nshell + s0 + c] = tile[r][c];
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void multalm_kernel(int64_t n, const double2 *__restrict__ alm,
                                const double *__restrict__ bl, double2 *__restrict__ out) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -251,6 +315,13 @@ extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t
   const int64_t nalm = int64_t(nl) * (nl + 1) / 2;
   const int64_t n = nalm * nshell;
   const dim3 grid(nblocks(n, 256)), block(256);
+  if (!zring && out_f32 && k + 1 <= kWinMax) {
+    hipLaunchKernelGGL(fields_alm_window_kernel, dim3(nblocks(nalm, kFaBlock)), dim3(kFaBlock),
+                       0, (hipStream_t)stream, nl, k, first_shell, nshell, nalm, w, seed,
+                       reinterpret_cast<float2 *>(out_f32));
+    GA_LAUNCH_CHECK();
+    return GA_OK;
+  }
   if (zring)
     hipLaunchKernelGGL(fields_alm_kernel<true>, grid, block, 0, (hipStream_t)stream, nl, k,
                        first_shell, nshell, nalm, w, reinterpret_cast<const double2 *>(zring),

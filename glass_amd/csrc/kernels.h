@@ -11,11 +11,14 @@ int launch_legendre_synth_f64(const ga_plan *p, const double *alm, double *F, hi
 // Legendre stage, FP32 MFMA batched: alm [idx][S] complex64 -> F [ring][m][S] complex64
 int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float *F, hipStream_t s);
 
-// Ring stage of synthesis: F [ring][m][S] -> maps [S][npix] with fused transform
-int launch_ring_synth_f64(const ga_plan *p, int S, const double *F, double *maps,
+// Ring stage of synthesis: F -> maps [S][npix] with fused transform.  F holds
+// the shells in groups of G: offset ((ring ngrp + s/G)(lmax+1) + m) G + s%G
+// (G = S is the plain [ring][m][S] layout; the MFMA stage writes G = 8).
+int launch_ring_synth_f64(const ga_plan *p, int S, int G, const double *F, double *maps,
                           const ga_epilogue *epi, hipStream_t s);
-int launch_ring_synth_f32(const ga_plan *p, int S, const float *F, float *maps,
+int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
                           const ga_epilogue *epi, hipStream_t s);
+constexpr int kMfmaGroup = 4;  // shells per group in the MFMA stage's ring buffer
 
 // Ring stage of analysis: maps [S][npix] -> F [ring][m][S] (weighted, phase removed)
 int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s);

@@ -164,6 +164,7 @@ __global__ __launch_bounds__(256) void spin_synth_kernel(
     const double *__restrict__ sth, const double *__restrict__ tab /* (A,B,C) per (l,m) */,
     const double2 *__restrict__ seedk /* per m: (+s mant, +s exp), */,
     const double2 *__restrict__ seedkm /* per m: (-s mant, -s exp) */,
+    const int32_t *__restrict__ ring_pnum, const int32_t *__restrict__ ring_pden,
     const cplx<T> *__restrict__ E, const cplx<T> *__restrict__ B, cplx<T> *__restrict__ Fo) {
   const int m = blockIdx.x;
   const int rp = blockIdx.y * blockDim.x + threadIdx.x;
@@ -229,6 +230,11 @@ __global__ __launch_bounds__(256) void spin_synth_kernel(
     }
   }
   if (rp >= npairs) return;
+  // ring phase e^{i m phi0} applied here (the same on the south mirror)
+  const int pnum = ring_pnum[rp], pden = ring_pden[rp];
+  double ps = 0.0, pc = 1.0;
+  if (pnum) sincospi(2.0 * double((int64_t(m) * pnum) % pden) / double(pden), &ps, &pc);
+  const cplx<T> ph = {T(pc), T(ps)};
   // output ring buffer [ring][m][2S]: column 2s holds Q_s, 2s + 1 holds U_s
   const int64_t rowN = (int64_t(rp) * (lmax + 1) + m) * (2 * S) + 2 * s0;
   const int64_t rowS = (int64_t(nrings - 1 - rp) * (lmax + 1) + m) * (2 * S) + 2 * s0;
@@ -237,11 +243,11 @@ __global__ __launch_bounds__(256) void spin_synth_kernel(
     if (s0 + c >= S) break;
     const cplx<T> qa_e = acc[c][0][0], qb_e = acc[c][1][0], qa_o = acc[c][0][1], qb_o = acc[c][1][1];
     const cplx<T> ua_e = acc[c][2][0], ub_e = acc[c][3][0], ua_o = acc[c][2][1], ub_o = acc[c][3][1];
-    Fo[rowN + 2 * c] = cadd(cadd(qa_e, qb_e), cadd(qa_o, qb_o));
-    Fo[rowN + 2 * c + 1] = cadd(cadd(ua_e, ub_e), cadd(ua_o, ub_o));
+    Fo[rowN + 2 * c] = cmul(cadd(cadd(qa_e, qb_e), cadd(qa_o, qb_o)), ph);
+    Fo[rowN + 2 * c + 1] = cmul(cadd(cadd(ua_e, ub_e), cadd(ua_o, ub_o)), ph);
     if (rp < npairs - 1) {
-      Fo[rowS + 2 * c] = cadd(csub(qa_e, qb_e), csub(qb_o, qa_o));
-      Fo[rowS + 2 * c + 1] = cadd(csub(ua_e, ub_e), csub(ub_o, ua_o));
+      Fo[rowS + 2 * c] = cmul(cadd(csub(qa_e, qb_e), csub(qb_o, qa_o)), ph);
+      Fo[rowS + 2 * c + 1] = cmul(cadd(csub(ua_e, ub_e), csub(ub_o, ua_o)), ph);
     }
   }
 }
@@ -286,8 +292,8 @@ static int launch_spin(const ga_plan *p, int spin, int S, const T *E, const T *B
   dim3 grid(unsigned(p->lmax + 1), unsigned((p->npairs + 255) / 256), unsigned((S + SC - 1) / SC));
   hipLaunchKernelGGL((spin_synth_kernel<T, SC>), grid, dim3(256), 0, stream, p->lmax, p->npairs,
                      p->nrings, spin, S, p->d_z, p->d_sth, p->d_spin[spin],
-                     p->d_spin_seed[spin], p->d_spin_seed[spin] + (p->lmax + 1),
-                     (const cplx<T> *)E, (const cplx<T> *)B, (cplx<T> *)Fo);
+                     p->d_spin_seed[spin], p->d_spin_seed[spin] + (p->lmax + 1), p->d_ring_pnum,
+                     p->d_ring_pden, (const cplx<T> *)E, (const cplx<T> *)B, (cplx<T> *)Fo);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

@@ -19,7 +19,9 @@ namespace {
 __global__ __launch_bounds__(256) void legendre_synth_f64_kernel(
     int lmax, int npairs, int nrings, const double *__restrict__ z,
     const double *__restrict__ sth, const double2 *__restrict__ ab,
-    const double *__restrict__ cm, const double2 *__restrict__ alm, double2 *__restrict__ F) {
+    const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
+    const int32_t *__restrict__ ring_pden, const double2 *__restrict__ alm,
+    double2 *__restrict__ F) {
   const int m = blockIdx.x;
   const int rp = blockIdx.y * blockDim.x + threadIdx.x;
   const int rpc = rp < npairs ? rp : npairs - 1;
@@ -91,9 +93,15 @@ __global__ __launch_bounds__(256) void legendre_synth_f64_kernel(
   }
   if (m == 0) { ei = 0.0; oi = 0.0; }  // Im a_l0 is ignored (real map)
   if (rp >= npairs) return;
+  // ring phase e^{i m phi0}, phi0 = 2 pi pnum / pden (the same on the south mirror)
+  const int pnum = ring_pnum[rp], pden = ring_pden[rp];
+  double ps = 0.0, pc = 1.0;
+  if (pnum) sincospi(2.0 * double((int64_t(m) * pnum) % pden) / double(pden), &ps, &pc);
   const int64_t stride = lmax + 1;
-  F[int64_t(rp) * stride + m] = double2{er + orr, ei + oi};
-  if (rp < npairs - 1) F[int64_t(nrings - 1 - rp) * stride + m] = double2{er - orr, ei - oi};
+  const double nr = er + orr, ni = ei + oi, sr = er - orr, si = ei - oi;
+  F[int64_t(rp) * stride + m] = double2{nr * pc - ni * ps, nr * ps + ni * pc};
+  if (rp < npairs - 1)
+    F[int64_t(nrings - 1 - rp) * stride + m] = double2{sr * pc - si * ps, sr * ps + si * pc};
 }
 
 }  // namespace
@@ -102,8 +110,9 @@ int launch_legendre_synth_f64(const ga_plan *p, const double *alm, double *F,
                               hipStream_t stream) {
   dim3 grid(unsigned(p->lmax + 1), unsigned((p->npairs + 255) / 256));
   hipLaunchKernelGGL(legendre_synth_f64_kernel, grid, dim3(256), 0, stream, p->lmax, p->npairs,
-                     p->nrings, p->d_z, p->d_sth, p->d_ab, p->d_cm,
-                     reinterpret_cast<const double2 *>(alm), reinterpret_cast<double2 *>(F));
+                     p->nrings, p->d_z, p->d_sth, p->d_ab, p->d_cm, p->d_ring_pnum,
+                     p->d_ring_pden, reinterpret_cast<const double2 *>(alm),
+                     reinterpret_cast<double2 *>(F));
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

@@ -32,11 +32,15 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ float u2f(unsigned u) { return __uint_as_float(u); }
 __device__ __forceinline__ unsigned f2u(float f) { return __float_as_uint(f); }
 
-template <int NT>
-__global__ __launch_bounds__(256, 2) void legendre_synth_mfma_kernel(
+// CHUNK > 0: the MFMA accumulators are flushed into FP32 master sums every
+// CHUNK quads, so each rounding acts on a partial sum of at most 2 CHUNK terms
+// per parity instead of the full l-range (error ~ sqrt(K) -> ~K^(1/4)).
+template <int NT, int CHUNK, int MINW>
+__global__ __launch_bounds__(256, MINW) void legendre_synth_mfma_kernel(
     int lmax, int npairs, int nrings, int ncols, int nrg, int ncg,
     const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
-    const double *__restrict__ cm, const float *__restrict__ alm, float *__restrict__ F) {
+    const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
+    const int32_t *__restrict__ ring_pden, const float *__restrict__ alm, float *__restrict__ F) {
   // XCD-aware decode: block b runs on XCD b % 8; XCD x takes m = x, x + 8, ...
   // (largest l-range first), all (ring group, column group) tiles of one m
   // back to back so they share that m's alm slice in the XCD's L2.
@@ -60,11 +64,17 @@ __global__ __launch_bounds__(256, 2) void legendre_synth_mfma_kernel(
   const double2 *__restrict__ co = ab + base;
   const int half = lane >> 5, col_in = lane & 31;
   const int colbase = cg * 32 * NT;
-  const float *__restrict__ bptr = alm + base * ncols + colbase + col_in;
-  bool colok[NT];
+  // per-lane constant offsets of the B elements inside a quad's rows (rows l0 + p + 2 half):
+  // columns past ncols read row-start data instead (their outputs are never
+  // stored), so every load is in bounds and needs no mask
+  int boff[2][NT];
 #pragma unroll
-  for (int ct = 0; ct < NT; ++ct)
-    colok[ct] = (colbase + ct * 32 + col_in < ncols) && !(m == 0 && (col_in & 1));
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+      const int col = colbase + ct * 32 + col_in;
+      boff[p][ct] = (col < ncols) ? (p + 2 * half) * ncols + col : (p + 2 * half) * ncols;
+    }
 
   f32x16 acc[2][2][NT];  // [parity][row tile][col tile]
 #pragma unroll
@@ -73,19 +83,42 @@ __global__ __launch_bounds__(256, 2) void legendre_synth_mfma_kernel(
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
+  f32x16 mst[CHUNK > 0 ? 2 : 1][2][NT];
+  if constexpr (CHUNK > 0) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) mst[p][t][ct] = f32x16{};
+  }
+  int nq = 0;
 
-  // One loop body for every quad so the accumulators stay in place.  `check`
-  // (wave-uniform) is true while some lane still carries a scaled value below
-  // the representable range; the MFMAs are skipped while no lane is in range.
+  // B rows l0 .. l0 + 3 start at a wave-uniform address; in a tail quad rows
+  // past lmax are replaced by zeros (lanes whose row is out of range load the
+  // quad's first row, which exists).
+  auto load_b = [&](int l0n, float (&bo)[2][NT]) {
+    const float *__restrict__ row = alm + (base + (l0n - m)) * int64_t(ncols);
+    const bool full = l0n + 3 <= lmax;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const bool lok = full || (l0n + p + 2 * half <= lmax);
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const float v = row[lok ? boff[p][ct] : 0];
+        bo[p][ct] = lok ? v : 0.0f;
+      }
+    }
+  };
+  // one quad: 4 recurrence steps (coefficients cq), MFMAs with operands bq
   bool check = true;
-  for (int l0 = m; l0 <= lmax; l0 += 4) {
+  auto quad = [&](int l0, const double2 (&cq)[4], const float (&bq)[2][NT]) {
     const bool tail = l0 + 3 > lmax;
     float f[4];
     if (check) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double2 c = co[l0 + j - m];
-        const double pn = c.x * (x * p1 - c.y * p0);
+        const double pn = cq[j].x * (x * p1 - cq[j].y * p0);
         p0 = p1;
         p1 = pn;
         if (k < 0 && fabs(p1) > 0x1p300) { p1 *= 0x1p-600; p0 *= 0x1p-600; ++k; }
@@ -94,8 +127,7 @@ __global__ __launch_bounds__(256, 2) void legendre_synth_mfma_kernel(
     } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double2 c = co[l0 + j - m];
-        const double pn = c.x * (x * p1 - c.y * p0);
+        const double pn = cq[j].x * (x * p1 - cq[j].y * p0);
         p0 = p1;
         p1 = pn;
         f[j] = float(p1);
@@ -108,40 +140,106 @@ __global__ __launch_bounds__(256, 2) void legendre_synth_mfma_kernel(
     }
     const bool on = !check || wave_any(k == 0);
     if (check) check = wave_any(k < 0);
-    if (!on) continue;
-    const auto sw0 = __builtin_amdgcn_permlane32_swap(f2u(f[0]), f2u(f[2]), false, false);
-    const auto sw1 = __builtin_amdgcn_permlane32_swap(f2u(f[1]), f2u(f[3]), false, false);
-    const float av[2][2] = {{u2f(sw0[0]), u2f(sw0[1])}, {u2f(sw1[0]), u2f(sw1[1])}};
+    if (on) {
+      const auto sw0 = __builtin_amdgcn_permlane32_swap(f2u(f[0]), f2u(f[2]), false, false);
+      const auto sw1 = __builtin_amdgcn_permlane32_swap(f2u(f[1]), f2u(f[3]), false, false);
+      const float av[2][2] = {{u2f(sw0[0]), u2f(sw0[1])}, {u2f(sw1[0]), u2f(sw1[1])}};
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int l = l0 + p + 2 * half;
-      const bool lok = l <= lmax;
+      for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < NT; ++ct) {
-        const float bv = (lok && colok[ct]) ? bptr[int64_t(l - m) * ncols + ct * 32] : 0.0f;
-        acc[p][0][ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[p][0], bv, acc[p][0][ct], 0, 0, 0);
-        acc[p][1][ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[p][1], bv, acc[p][1][ct], 0, 0, 0);
+        for (int ct = 0; ct < NT; ++ct) {
+          acc[p][0][ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[p][0], bq[p][ct], acc[p][0][ct], 0, 0, 0);
+          acc[p][1][ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[p][1], bq[p][ct], acc[p][1][ct], 0, 0, 0);
+        }
+    }
+    if constexpr (CHUNK > 0) {
+      if (++nq == CHUNK) {
+        nq = 0;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+              mst[p][t][ct] += acc[p][t][ct];
+              acc[p][t][ct] = f32x16{};
+            }
       }
     }
+  };
+  auto load_c = [&](int l0n, double2 (&cq)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cq[j] = co[l0n + j - m];
+  };
+  // two quads per iteration, operands of the next quad loaded before the
+  // current one is computed (ping-pong buffers A / B, no copies)
+  float bA[2][NT], bB[2][NT];
+  double2 cA[4], cB[4];
+  load_b(m, bA);
+  load_c(m, cA);
+  int l0 = m;
+  for (;;) {
+    const bool hasB = l0 + 4 <= lmax;
+    if (hasB) { load_b(l0 + 4, bB); load_c(l0 + 4, cB); }
+    quad(l0, cA, bA);
+    if (!hasB) break;
+    const bool hasA = l0 + 8 <= lmax;
+    if (hasA) { load_b(l0 + 8, bA); load_c(l0 + 8, cA); }
+    quad(l0 + 4, cB, bB);
+    if (!hasA) break;
+    l0 += 8;
+  }
+  if constexpr (CHUNK > 0) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] += mst[p][t][ct];
   }
 
   // epilogue: D layout of 32x32x2: lane L, acc element e -> row 8 (e/4) + 4 (L/32) + e%4,
-  // column L % 32.  north = E + O, south = E - O.
-  const int64_t mstride = int64_t(lmax + 1) * ncols;
+  // column L % 32.  north = E + O, south = E - O, both times the ring phase
+  // e^{i m phi0} (equal for a ring and its south mirror).  Each lane computes
+  // the phase of its own ring (FP64, exact argument reduction); rows fetch it
+  // by shuffle, and the (re, im) columns of a shell sit in lanes (2c, 2c+1).
+  // Ring buffer layout: real column c (shell c/2) of ring r lives at
+  //   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8),
+  // i.e. shells in groups of 4, each (ring, group, m) row 32 contiguous bytes.
+  float phc, phs;
+  {
+    const int pnum = ring_pnum[rpc], pden = ring_pden[rpc];
+    double ps = 0.0, pc = 1.0;
+    if (pnum) sincospi(2.0 * double((int64_t(m) * pnum) % pden) / double(pden), &ps, &pc);
+    phc = float(pc);
+    phs = float(ps);
+  }
+  const int ngrp = (ncols + 7) / 8;
+  const bool is_im = (col_in & 1) != 0;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
 #pragma unroll
-    for (int ct = 0; ct < NT; ++ct) {
-      const int col = colbase + ct * 32 + col_in;
-      if (col >= ncols) continue;
+    for (int e = 0; e < 16; ++e) {
+      const int row = 32 * t + 8 * (e >> 2) + 4 * half + (e & 3);
+      const float c = __shfl(phc, row, 64), sn = __shfl(phs, row, 64);
+      const int rpo = rp0 + row;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int row = 8 * (e >> 2) + 4 * half + (e & 3);
-        const int rpo = rp0 + 32 * t + row;
-        if (rpo >= npairs) continue;
+      for (int ct = 0; ct < NT; ++ct) {
+        const int col = colbase + ct * 32 + col_in;
         const float ev = acc[0][t][ct][e], ov = acc[1][t][ct][e];
-        F[int64_t(rpo) * mstride + int64_t(m) * ncols + col] = ev + ov;
-        if (rpo < npairs - 1) F[int64_t(nrings - 1 - rpo) * mstride + int64_t(m) * ncols + col] = ev - ov;
+        const float nv = ev + ov, sv = ev - ov;
+        const float no = __shfl_xor(nv, 1, 64), so = __shfl_xor(sv, 1, 64);
+        // re' = re c - im s,  im' = re s + im c
+        const float nrot = is_im ? (no * sn + nv * c) : (nv * c - no * sn);
+        const float srot = is_im ? (so * sn + sv * c) : (sv * c - so * sn);
+        if (rpo < npairs && col < ncols) {
+          const float nrz = (m == 0 && is_im) ? 0.0f : nrot;
+          const float srz = (m == 0 && is_im) ? 0.0f : srot;
+          const int64_t gofs = (int64_t(col >> 3) * (lmax + 1) + m) * 8 + (col & 7);
+          F[int64_t(rpo) * ngrp * (lmax + 1) * 8 + gofs] = nrz;
+          if (rpo < npairs - 1)
+            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * 8 + gofs] = srz;
+        }
       }
     }
   }
@@ -152,20 +250,31 @@ __global__ __launch_bounds__(256, 2) void legendre_synth_mfma_kernel(
 int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float *F,
                                hipStream_t stream) {
   const int ncols = 2 * S;
-  const int nt = ncols <= 32 ? 1 : 2;
+  const ga_tuning &tu = tuning();
+  int nt = tu.mfma_nt;
+  if (nt != 1 && nt != 2) nt = ncols <= 32 ? 1 : 2;
+  const int chunk = tu.mfma_chunk;
   const int nrg = (p->npairs + 255) / 256;
   const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
   const int nm8 = (p->lmax + 1 + 7) / 8;
   const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
   GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
-  if (nt == 1)
-    hipLaunchKernelGGL(legendre_synth_mfma_kernel<1>, dim3(unsigned(nblk)), dim3(256), 0, stream,
-                       p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth, p->d_ab,
-                       p->d_cm, alm, F);
-  else
-    hipLaunchKernelGGL(legendre_synth_mfma_kernel<2>, dim3(unsigned(nblk)), dim3(256), 0, stream,
-                       p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth, p->d_ab,
-                       p->d_cm, alm, F);
+#define GA_MFMA_LAUNCH(NT_, CH_, MW_)                                                         \
+  hipLaunchKernelGGL((legendre_synth_mfma_kernel<NT_, CH_, MW_>), dim3(unsigned(nblk)),       \
+                     dim3(256), 0, stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg,    \
+                     p->d_z, p->d_sth, p->d_ab, p->d_cm, p->d_ring_pnum, p->d_ring_pden, alm, F)
+  if (nt == 1) {
+    if (chunk <= 0) GA_MFMA_LAUNCH(1, 0, 2);
+    else if (chunk <= 8) GA_MFMA_LAUNCH(1, 8, 2);
+    else if (chunk <= 16) GA_MFMA_LAUNCH(1, 16, 2);
+    else GA_MFMA_LAUNCH(1, 32, 2);
+  } else {
+    if (chunk <= 0) GA_MFMA_LAUNCH(2, 0, 2);
+    else if (chunk <= 8) GA_MFMA_LAUNCH(2, 8, 1);
+    else if (chunk <= 16) GA_MFMA_LAUNCH(2, 16, 1);
+    else GA_MFMA_LAUNCH(2, 32, 1);
+  }
+#undef GA_MFMA_LAUNCH
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

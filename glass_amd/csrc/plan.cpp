@@ -30,6 +30,11 @@ void set_error(const char *fmt, ...) {
 }
 void clear_error() { g_err.clear(); }
 
+ga_tuning &tuning() {
+  static ga_tuning t;
+  return t;
+}
+
 namespace {
 
 struct RingGeom {
@@ -172,6 +177,32 @@ using namespace ga;
 
 extern "C" const char *ga_last_error(void) { return g_err.c_str(); }
 extern "C" const char *ga_version(void) { return "glass_amd 0.1 (gfx950)"; }
+
+extern "C" int ga_set_tuning(int32_t key, int32_t value) {
+  clear_error();
+  switch (key) {
+    case GA_TUNE_MFMA_TILES:
+      GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "mfma tiles must be 0, 1 or 2");
+      tuning().mfma_nt = value;
+      return GA_OK;
+    case GA_TUNE_MFMA_CHUNK:
+      GA_REQUIRE(value >= 0 && value <= 4096, GA_ERR_INVALID, "mfma chunk out of range");
+      tuning().mfma_chunk = value;
+      return GA_OK;
+    default:
+      GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
+  }
+}
+
+extern "C" int ga_get_tuning(int32_t key, int32_t *value) {
+  clear_error();
+  GA_REQUIRE(value, GA_ERR_INVALID, "NULL value");
+  switch (key) {
+    case GA_TUNE_MFMA_TILES: *value = tuning().mfma_nt; return GA_OK;
+    case GA_TUNE_MFMA_CHUNK: *value = tuning().mfma_chunk; return GA_OK;
+    default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
+  }
+}
 
 extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
   clear_error();
@@ -326,6 +357,8 @@ extern "C" int ga_workspace_size(const ga_plan *p, int32_t nbatch, int32_t real_
   GA_REQUIRE(p && bytes, GA_ERR_INVALID, "NULL argument");
   GA_REQUIRE(nbatch >= 1, GA_ERR_INVALID, "nbatch must be >= 1");
   GA_REQUIRE(real_bytes == 4 || real_bytes == 8, GA_ERR_INVALID, "real_bytes must be 4 or 8");
-  *bytes = size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nbatch) * 2 * size_t(real_bytes);
+  // fp32 ring buffers hold whole groups of 8 shells (the MFMA stage layout)
+  const int nb = real_bytes == 4 ? (nbatch + 7) / 8 * 8 : nbatch;
+  *bytes = size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nb) * 2 * size_t(real_bytes);
   return GA_OK;
 }

@@ -51,6 +51,12 @@ __device__ __forceinline__ cplx<T> ld(const typename tab_t<T>::type *p, int64_t 
 
 __device__ __forceinline__ int ilog2(int x) { return 31 - __clz(x); }
 
+// LDS work arrays insert one pad element after every 32 (8-byte complex floats
+// span all 64 banks every 32 elements), which removes the bank conflicts of
+// the short-span butterflies and of the bit-reversed scatter.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 5); }
+__host__ __device__ inline int padded_len(int n) { return n + (n >> 5) + 1; }
+
 __device__ __forceinline__ cplx<float> expi_q(int64_t q, int64_t d, float) {
   float s, c;
   sincospif(2.0f * float(q) / float(d), &s, &c);
@@ -62,76 +68,91 @@ __device__ __forceinline__ cplx<double> expi_q(int64_t q, int64_t d, double) {
   return {c, s};
 }
 
-// In-place forward FFT (e^{-2 pi i}), natural order in, bit-reversed out.
-template <typename T, int NT>
-__device__ void fft_dif(cplx<T> *X, int L, const typename tab_t<T>::type *tw, int maxL) {
+// In-place forward FFT (e^{-2 pi i}), natural order in, bit-reversed out, on P
+// arrays X + p ldx at once (one barrier per stage for all of them).
+template <typename T, int NT, int P>
+__device__ void fft_dif(cplx<T> *X, int ldx, int L, const typename tab_t<T>::type *tw, int maxL) {
   const int t = threadIdx.x;
   const int logL = ilog2(L);
   int h = L >> 1;
   if (logL & 1) {
     const int stride = maxL / (2 * h);
-    for (int j = t; j < h; j += NT) {
+    const int lh = logL - 1;  // h = 2^lh
+    for (int jj = t; jj < h * P; jj += NT) {
+      const int p = jj >> lh, j = jj & (h - 1);
+      cplx<T> *Y = X + p * ldx;
       cplx<T> w = cconj(ld<T>(tw, int64_t(j) * stride));
-      cplx<T> u = X[j], v = X[j + h];
-      X[j] = cadd(u, v);
-      X[j + h] = cmul(csub(u, v), w);
+      cplx<T> u = Y[pad(j)], v = Y[pad(j + h)];
+      Y[pad(j)] = cadd(u, v);
+      Y[pad(j + h)] = cmul(csub(u, v), w);
     }
     __syncthreads();
     h >>= 1;
   }
+  const int L4 = L >> 2, lL4 = logL - 2;
   for (; h >= 2; h >>= 2) {
-    const int q = h >> 1;
+    const int q = h >> 1, lq = ilog2(q);
     const int stride = maxL / (2 * h);
-    for (int g = t; g < (L >> 2); g += NT) {
-      const int blk = g / q, pos = g - blk * q;
+    for (int gg = t; gg < L4 * P; gg += NT) {
+      const int p = gg >> lL4, g = gg & (L4 - 1);
+      cplx<T> *Y = X + p * ldx;
+      const int blk = g >> lq, pos = g & (q - 1);
       const int b0 = blk * 2 * h + pos;
       cplx<T> w1 = cconj(ld<T>(tw, int64_t(pos) * stride));
       cplx<T> w2 = cmul(w1, w1);
       cplx<T> w3 = {w1.im, -w1.re};  // w1 * (-i)
-      cplx<T> a = X[b0], b = X[b0 + q], c = X[b0 + h], d = X[b0 + h + q];
+      cplx<T> a = Y[pad(b0)], b = Y[pad(b0 + q)], c = Y[pad(b0 + h)], d = Y[pad(b0 + h + q)];
       cplx<T> a1 = cadd(a, c), c1 = cmul(csub(a, c), w1);
       cplx<T> b1 = cadd(b, d), d1 = cmul(csub(b, d), w3);
-      X[b0] = cadd(a1, b1);
-      X[b0 + q] = cmul(csub(a1, b1), w2);
-      X[b0 + h] = cadd(c1, d1);
-      X[b0 + h + q] = cmul(csub(c1, d1), w2);
+      Y[pad(b0)] = cadd(a1, b1);
+      Y[pad(b0 + q)] = cmul(csub(a1, b1), w2);
+      Y[pad(b0 + h)] = cadd(c1, d1);
+      Y[pad(b0 + h + q)] = cmul(csub(c1, d1), w2);
     }
     __syncthreads();
   }
 }
 
-// In-place inverse FFT (e^{+2 pi i}, unscaled), bit-reversed in, natural out.
-template <typename T, int NT>
-__device__ void fft_dit(cplx<T> *X, int L, const typename tab_t<T>::type *tw, int maxL) {
+// In-place inverse FFT (e^{+2 pi i}, unscaled), bit-reversed in, natural out,
+// on P arrays X + p ldx.
+template <typename T, int NT, int P>
+__device__ void fft_dit(cplx<T> *X, int ldx, int L, const typename tab_t<T>::type *tw, int maxL) {
   const int t = threadIdx.x;
   const int logL = ilog2(L);
   int h = 1;
   if (logL & 1) {
-    for (int j = t; j < (L >> 1); j += NT) {
-      cplx<T> u = X[2 * j], v = X[2 * j + 1];
-      X[2 * j] = cadd(u, v);
-      X[2 * j + 1] = csub(u, v);
+    const int L2 = L >> 1, lL2 = logL - 1;
+    for (int jj = t; jj < L2 * P; jj += NT) {
+      const int p = jj >> lL2, j = jj & (L2 - 1);
+      cplx<T> *Y = X + p * ldx;
+      cplx<T> u = Y[pad(2 * j)], v = Y[pad(2 * j + 1)];
+      Y[pad(2 * j)] = cadd(u, v);
+      Y[pad(2 * j + 1)] = csub(u, v);
     }
     __syncthreads();
     h = 2;
   }
+  const int L4 = L >> 2, lL4 = logL - 2;
   for (; h < L; h <<= 2) {
     const int stride = maxL / (4 * h);
-    for (int g = t; g < (L >> 2); g += NT) {
-      const int blk = g / h, pos = g - blk * h;
+    const int lh = ilog2(h);
+    for (int gg = t; gg < L4 * P; gg += NT) {
+      const int p = gg >> lL4, g = gg & (L4 - 1);
+      cplx<T> *Y = X + p * ldx;
+      const int blk = g >> lh, pos = g & (h - 1);
       const int a0 = blk * 4 * h + pos;
       cplx<T> w4 = ld<T>(tw, int64_t(pos) * stride);
       cplx<T> w = cmul(w4, w4);
       cplx<T> w4i = cmuli(w4);
-      cplx<T> a = X[a0], b = X[a0 + h], c = X[a0 + 2 * h], d = X[a0 + 3 * h];
+      cplx<T> a = Y[pad(a0)], b = Y[pad(a0 + h)], c = Y[pad(a0 + 2 * h)], d = Y[pad(a0 + 3 * h)];
       cplx<T> tb = cmul(b, w), td = cmul(d, w);
       cplx<T> a1 = cadd(a, tb), b1 = csub(a, tb);
       cplx<T> c1 = cadd(c, td), d1 = csub(c, td);
       cplx<T> tc = cmul(c1, w4), tdd = cmul(d1, w4i);
-      X[a0] = cadd(a1, tc);
-      X[a0 + 2 * h] = csub(a1, tc);
-      X[a0 + h] = cadd(b1, tdd);
-      X[a0 + 3 * h] = csub(b1, tdd);
+      Y[pad(a0)] = cadd(a1, tc);
+      Y[pad(a0 + 2 * h)] = csub(a1, tc);
+      Y[pad(a0 + h)] = cadd(b1, tdd);
+      Y[pad(a0 + 3 * h)] = csub(b1, tdd);
     }
     __syncthreads();
   }
@@ -143,13 +164,13 @@ __device__ __forceinline__ int blu_len(int r) {
   return L;
 }
 
-// Inverse length-r DFT of the values H[i][slot] (owned j1 = t + i NT), in place
-// in the registers; X is the LDS work array.  conj_io conjugates input and
-// output, which turns it into the forward DFT.
-template <typename T, int NT, int MAXC>
-__device__ void sub_idft(cplx<T> (&H)[MAXC][4], int slot, int r, cplx<T> *X, int64_t boff,
-                         const typename tab_t<T>::type *blu, const typename tab_t<T>::type *tw,
-                         int maxL, bool conj_io) {
+// Inverse length-r DFT of the values H[i][slot][p] (owned j1 = t + i NT) for
+// the P arrays p, in place in the registers; X holds P work arrays of stride
+// ldx in LDS.  conj_io conjugates input and output (the forward DFT).
+template <typename T, int NT, int MAXC, int P>
+__device__ void sub_idft(cplx<T> (&H)[MAXC][4][P], int slot, int r, cplx<T> *X, int ldx,
+                         int64_t boff, const typename tab_t<T>::type *blu,
+                         const typename tab_t<T>::type *tw, int maxL, bool conj_io) {
   const int t = threadIdx.x;
   __syncthreads();  // X free
   if (boff < 0) {   // r is a power of two
@@ -158,20 +179,25 @@ __device__ void sub_idft(cplx<T> (&H)[MAXC][4], int slot, int r, cplx<T> *X, int
     for (int i = 0; i < MAXC; ++i) {
       const int j1 = t + i * NT;
       if (j1 < r) {
-        cplx<T> v = H[i][slot];
-        if (conj_io) v = cconj(v);
         const int pos = bits ? int(__brev(unsigned(j1)) >> (32 - bits)) : 0;
-        X[pos] = v;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          cplx<T> v = H[i][slot][p];
+          X[p * ldx + pad(pos)] = conj_io ? cconj(v) : v;
+        }
       }
     }
     __syncthreads();
-    fft_dit<T, NT>(X, r, tw, maxL);
+    fft_dit<T, NT, P>(X, ldx, r, tw, maxL);
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       const int j1 = t + i * NT;
       if (j1 < r) {
-        cplx<T> v = X[j1];
-        H[i][slot] = conj_io ? cconj(v) : v;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          cplx<T> v = X[p * ldx + pad(j1)];
+          H[i][slot][p] = conj_io ? cconj(v) : v;
+        }
       }
     }
   } else {
@@ -182,23 +208,38 @@ __device__ void sub_idft(cplx<T> (&H)[MAXC][4], int slot, int r, cplx<T> *X, int
     for (int i = 0; i < MAXC; ++i) {
       const int j1 = t + i * NT;
       if (j1 < r) {
-        cplx<T> v = H[i][slot];
-        if (conj_io) v = cconj(v);
-        X[j1] = cmul(v, ld<T>(c, j1));
+        const cplx<T> cj = ld<T>(c, j1);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          cplx<T> v = H[i][slot][p];
+          if (conj_io) v = cconj(v);
+          X[p * ldx + pad(j1)] = cmul(v, cj);
+        }
       }
     }
-    for (int j = r + t; j < L; j += NT) X[j] = cplx<T>{T(0), T(0)};
+    for (int jj = t; jj < (L - r) * P; jj += NT) {
+      const int p = jj / (L - r), j = r + jj - p * (L - r);
+      X[p * ldx + pad(j)] = cplx<T>{T(0), T(0)};
+    }
     __syncthreads();
-    fft_dif<T, NT>(X, L, tw, maxL);
-    for (int j = t; j < L; j += NT) X[j] = cmul(X[j], ld<T>(bh, j));
+    fft_dif<T, NT, P>(X, ldx, L, tw, maxL);
+    const int lL = ilog2(L);
+    for (int jj = t; jj < L * P; jj += NT) {
+      const int p = jj >> lL, j = jj & (L - 1);
+      X[p * ldx + pad(j)] = cmul(X[p * ldx + pad(j)], ld<T>(bh, j));
+    }
     __syncthreads();
-    fft_dit<T, NT>(X, L, tw, maxL);
+    fft_dit<T, NT, P>(X, ldx, L, tw, maxL);
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       const int j1 = t + i * NT;
       if (j1 < r) {
-        cplx<T> v = cmul(X[j1], ld<T>(c, j1));
-        H[i][slot] = conj_io ? cconj(v) : v;
+        const cplx<T> cj = ld<T>(c, j1);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          cplx<T> v = cmul(X[p * ldx + pad(j1)], cj);
+          H[i][slot][p] = conj_io ? cconj(v) : v;
+        }
       }
     }
   }
@@ -228,82 +269,113 @@ __device__ __forceinline__ bool decode_ring_block(int nrings, int npp, int &ring
   return ring < nrings;
 }
 
-template <typename T, int NT, int MAXC>
+// Ring buffer layout: shells in groups of G (complex), F offset of (ring, m, s)
+//   ((ring * ngrp + s / G) * (lmax + 1) + m) * G + s % G,   ngrp = ceil(S / G).
+// G = S is the plain [ring][m][S] layout.  The producers (Legendre stages)
+// store F_m e^{i m phi0}: the ring phase is already applied.
+// A workgroup synthesises a unit of 2P consecutive shells (P complex
+// transforms).  Split of the length n = 4 r inverse DFT used here
+// (decimation in frequency): with k = k1 + r k2 and j = 4 j1 + j2,
+//   x_{4 j1 + j2} = sum_{k1} w_r^{j1 k1} [ w_n^{j2 k1} sum_{k2} i^{j2 k2} H_{k1 + r k2} ],
+// so a thread owning k1 = t + i NT folds the bins k1, k1 + r, k1 + 2r, k1 + 3r
+// (neighbouring lanes read neighbouring m: coalesced), does the radix-4 step
+// and twiddle in registers, then the four length-r DFTs run in LDS and the
+// thread stores pixels 4 j1 .. 4 j1 + 3.
+template <typename T, int NT, int MAXC, int P>
 __global__ __launch_bounds__(NT) void ring_synth_kernel(
     int lmax, int nrings, const int64_t *__restrict__ ring_start,
-    const int32_t *__restrict__ ring_n, const int32_t *__restrict__ ring_pnum,
-    const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ blu_off,
+    const int32_t *__restrict__ ring_n, const int64_t *__restrict__ blu_off,
     const typename tab_t<T>::type *__restrict__ blu, const typename tab_t<T>::type *__restrict__ tw,
-    int maxL, const cplx<T> *__restrict__ F, int S, int npp, T *__restrict__ maps, int64_t npix,
-    const ga_epilogue *__restrict__ epi) {
+    int maxL, int ldx, const cplx<T> *__restrict__ F, int S, int G, int nunit,
+    T *__restrict__ maps, int64_t npix, const ga_epilogue *__restrict__ epi) {
   extern __shared__ char smem_raw[];
   cplx<T> *X = reinterpret_cast<cplx<T> *>(smem_raw);
-  int ring, pair;
-  if (!decode_ring_block(nrings, npp, ring, pair)) return;
+  int ring, unit;
+  if (!decode_ring_block(nrings, nunit, ring, unit)) return;
   const int t = threadIdx.x;
-  const int s0 = 2 * pair, s1 = s0 + 1;
-  const bool has1 = s1 < S;
+  const int sbase = 2 * P * unit;
   const int n = ring_n[ring], r = n >> 2;
-  const int pnum = ring_pnum[ring], pden = ring_pden[ring];
   const int64_t boff = blu_off[r];
-  const int64_t frow = int64_t(ring) * (lmax + 1);
+  const int ngrp = (S + G - 1) / G;
+  // pointer of shell sbase at m = 0; the unit's 2P shells are contiguous
+  // because G is a multiple of 2P or G = S
+  const cplx<T> *Fu = F + (int64_t(ring) * ngrp + sbase / G) * (lmax + 1) * G + sbase % G;
+  int nsh = S - sbase;
+  if (nsh > 2 * P) nsh = 2 * P;
 
-  cplx<T> H[MAXC][4];
-  // ---- fold coefficients into the n bins, bins k = 4 j1 + k2 owned by this thread
+  cplx<T> H[MAXC][4][P];
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
-    const int j1 = t + i * NT;
+    const int k1 = t + i * NT;
+    cplx<T> h[4][P];
 #pragma unroll
     for (int k2 = 0; k2 < 4; ++k2) {
-      cplx<T> g0 = {T(0), T(0)}, g1 = {T(0), T(0)};
-      if (j1 < r) {
-        const int k = 4 * j1 + k2;
+      cplx<T> g[2 * P];
+#pragma unroll
+      for (int q = 0; q < 2 * P; ++q) g[q] = cplx<T>{T(0), T(0)};
+      if (k1 < r) {
+        const int k = k1 + r * k2;
+        // G_k = sum_{m = k mod n} F_m + sum_{m = -k mod n, m > 0} conj(F_m)
         for (int m = k; m <= lmax; m += n) {
-          const cplx<T> ph = pnum ? expi_q(m % pden, pden, T(0)) : cplx<T>{T(1), T(0)};
-          const cplx<T> *f = F + (frow + m) * S;
-          g0 = cadd(g0, cmul(f[s0], ph));
-          if (has1) g1 = cadd(g1, cmul(f[s1], ph));
+          const cplx<T> *f = Fu + int64_t(m) * G;
+#pragma unroll
+          for (int q = 0; q < 2 * P; ++q)
+            if (q < nsh) g[q] = cadd(g[q], f[q]);
         }
         for (int m = (k == 0 ? n : n - k); m <= lmax; m += n) {
-          const cplx<T> ph = pnum ? expi_q(m % pden, pden, T(0)) : cplx<T>{T(1), T(0)};
-          const cplx<T> *f = F + (frow + m) * S;
-          g0 = cadd(g0, cconj(cmul(f[s0], ph)));
-          if (has1) g1 = cadd(g1, cconj(cmul(f[s1], ph)));
+          const cplx<T> *f = Fu + int64_t(m) * G;
+#pragma unroll
+          for (int q = 0; q < 2 * P; ++q)
+            if (q < nsh) g[q] = cadd(g[q], cconj(f[q]));
         }
       }
-      H[i][k2] = cplx<T>{g0.re - g1.im, g0.im + g1.re};
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        h[k2][p] = cplx<T>{g[2 * p].re - g[2 * p + 1].im, g[2 * p].im + g[2 * p + 1].re};
+    }
+    // radix-4 across k2 and twiddle w_n^{j2 k1} (k1 < r, so w2 = w1^2, w3 = w1^3)
+    const cplx<T> w1 = expi_q(k1 < r ? k1 : 0, n, T(0));
+    const cplx<T> w2 = cmul(w1, w1);
+    const cplx<T> w3 = cmul(w2, w1);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const cplx<T> s02 = cadd(h[0][p], h[2][p]), d02 = csub(h[0][p], h[2][p]);
+      const cplx<T> s13 = cadd(h[1][p], h[3][p]), d13 = cmuli(csub(h[1][p], h[3][p]));
+      H[i][0][p] = cadd(s02, s13);
+      H[i][1][p] = cmul(cadd(d02, d13), w1);
+      H[i][2][p] = cmul(csub(s02, s13), w2);
+      H[i][3][p] = cmul(csub(d02, d13), w3);
     }
   }
-  // ---- four length-r inverse DFTs
+  // ---- four length-r inverse DFTs per pair
 #pragma unroll
-  for (int k2 = 0; k2 < 4; ++k2) sub_idft<T, NT, MAXC>(H, k2, r, X, boff, blu, tw, maxL, false);
+  for (int j2 = 0; j2 < 4; ++j2)
+    sub_idft<T, NT, MAXC, P>(H, j2, r, X, ldx, boff, blu, tw, maxL, false);
 
-  // ---- radix-4 combine, epilogue, store
+  // ---- epilogue and store: pixel 4 j1 + j2 of each shell (epilogue
+  // parameters are workgroup-uniform loads, read where they are used)
   const int64_t st = ring_start[ring];
-  T lam0 = T(1), par0 = T(0), lam1 = T(1), par1 = T(0);
-  int kind0 = GA_T_NORMAL, kind1 = GA_T_NORMAL;
-  if (epi) {
-    kind0 = epi[s0].kind; lam0 = T(epi[s0].lamda); par0 = T(epi[s0].param);
-    if (has1) { kind1 = epi[s1].kind; lam1 = T(epi[s1].lamda); par1 = T(epi[s1].param); }
-  }
-  T *out0 = maps + int64_t(s0) * npix + st;
-  T *out1 = maps + int64_t(s1) * npix + st;
 #pragma unroll
-  for (int i = 0; i < MAXC; ++i) {
-    const int j1 = t + i * NT;
-    if (j1 < r) {
-      const cplx<T> y0 = H[i][0];
-      const cplx<T> y1 = cmul(H[i][1], expi_q(j1, n, T(0)));
-      const cplx<T> y2 = cmul(H[i][2], expi_q((2 * j1) % n, n, T(0)));
-      const cplx<T> y3 = cmul(H[i][3], expi_q((3 * j1) % n, n, T(0)));
-      const cplx<T> s02 = cadd(y0, y2), d02 = csub(y0, y2);
-      const cplx<T> s13 = cadd(y1, y3), d13 = cmuli(csub(y1, y3));
-      const cplx<T> o[4] = {cadd(s02, s13), cadd(d02, d13), csub(s02, s13), csub(d02, d13)};
+  for (int p = 0; p < P; ++p) {
+    const int q0 = 2 * p, q1 = 2 * p + 1;
+    int k0 = GA_T_NORMAL, kk1 = GA_T_NORMAL;
+    T l0 = T(1), a0 = T(0), l1 = T(1), a1 = T(0);
+    if (epi) {
+      if (q0 < nsh) { k0 = epi[sbase + q0].kind; l0 = T(epi[sbase + q0].lamda); a0 = T(epi[sbase + q0].param); }
+      if (q1 < nsh) { kk1 = epi[sbase + q1].kind; l1 = T(epi[sbase + q1].lamda); a1 = T(epi[sbase + q1].param); }
+    }
+    T *out0 = maps + int64_t(sbase + q0) * npix + st;
+    T *out1 = maps + int64_t(sbase + q1) * npix + st;
 #pragma unroll
-      for (int j2 = 0; j2 < 4; ++j2) {
-        const int64_t pix = j1 + int64_t(r) * j2;
-        out0[pix] = apply_epi<T>(o[j2].re, kind0, lam0, par0);
-        if (has1) out1[pix] = apply_epi<T>(o[j2].im, kind1, lam1, par1);
+    for (int i = 0; i < MAXC; ++i) {
+      const int j1 = t + i * NT;
+      if (j1 < r) {
+#pragma unroll
+        for (int j2 = 0; j2 < 4; ++j2) {
+          const int64_t pix = 4 * int64_t(j1) + j2;
+          if (q0 < nsh) out0[pix] = apply_epi<T>(H[i][j2][p].re, k0, l0, a0);
+          if (q1 < nsh) out1[pix] = apply_epi<T>(H[i][j2][p].im, kk1, l1, a1);
+        }
       }
     }
   }
@@ -333,7 +405,7 @@ __global__ __launch_bounds__(NT) void ring_anal_kernel(
   const T *in1 = maps + int64_t(s1) * npix + st;
 
   // x_{4 j1 + j2} = f0 + i f1; thread owns j1
-  cplx<T> H[MAXC][4];
+  cplx<T> H[MAXC][4][1];
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     const int j1 = t + i * NT;
@@ -344,28 +416,29 @@ __global__ __launch_bounds__(NT) void ring_anal_kernel(
         v.re = in0[4 * j1 + j2];
         v.im = has1 ? in1[4 * j1 + j2] : T(0);
       }
-      H[i][j2] = v;
+      H[i][j2][0] = v;
     }
   }
   // forward length-r DFTs of each decimated sequence
 #pragma unroll
-  for (int j2 = 0; j2 < 4; ++j2) sub_idft<T, NT, MAXC>(H, j2, r, X, boff, blu, tw, maxL, true);
+  for (int j2 = 0; j2 < 4; ++j2)
+    sub_idft<T, NT, MAXC, 1>(H, j2, r, X, 0, boff, blu, tw, maxL, true);
   __syncthreads();
   // X_{k1 + r k2} = sum_{j2} (-i)^{j2 k2} e^{-2 pi i j2 k1 / n} D_{j2}[k1] -> LDS
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     const int k1 = t + i * NT;
     if (k1 < r) {
-      const cplx<T> y0 = H[i][0];
-      const cplx<T> y1 = cmul(H[i][1], cconj(expi_q(k1, n, T(0))));
-      const cplx<T> y2 = cmul(H[i][2], cconj(expi_q((2 * k1) % n, n, T(0))));
-      const cplx<T> y3 = cmul(H[i][3], cconj(expi_q((3 * k1) % n, n, T(0))));
+      const cplx<T> y0 = H[i][0][0];
+      const cplx<T> y1 = cmul(H[i][1][0], cconj(expi_q(k1, n, T(0))));
+      const cplx<T> y2 = cmul(H[i][2][0], cconj(expi_q((2 * k1) % n, n, T(0))));
+      const cplx<T> y3 = cmul(H[i][3][0], cconj(expi_q((3 * k1) % n, n, T(0))));
       const cplx<T> s02 = cadd(y0, y2), d02 = csub(y0, y2);
       const cplx<T> s13 = cadd(y1, y3), d13 = cmuli(csub(y3, y1));  // -i (y1 - y3)
-      X[k1] = cadd(s02, s13);
-      X[k1 + r] = cadd(d02, d13);
-      X[k1 + 2 * r] = csub(s02, s13);
-      X[k1 + 3 * r] = csub(d02, d13);
+      X[pad(k1)] = cadd(s02, s13);
+      X[pad(k1 + r)] = cadd(d02, d13);
+      X[pad(k1 + 2 * r)] = csub(s02, s13);
+      X[pad(k1 + 3 * r)] = csub(d02, d13);
     }
   }
   __syncthreads();
@@ -373,7 +446,7 @@ __global__ __launch_bounds__(NT) void ring_anal_kernel(
   for (int m = t; m <= lmax; m += NT) {
     const int k = m % n;
     const int kk = k ? n - k : 0;
-    const cplx<T> a = X[k], b = cconj(X[kk]);
+    const cplx<T> a = X[pad(k)], b = cconj(X[pad(kk)]);
     cplx<T> d0 = {T(0.5) * (a.re + b.re), T(0.5) * (a.im + b.im)};
     cplx<T> d1 = {T(0.5) * (a.im - b.im), T(-0.5) * (a.re - b.re)};  // (a - b) / (2i)
     const cplx<T> ph = pnum ? cconj(expi_q(m % pden, pden, T(0))) : cplx<T>{T(1), T(0)};
@@ -388,68 +461,96 @@ __global__ __launch_bounds__(NT) void ring_anal_kernel(
 }
 
 template <typename T>
-int launch_ring(const ga_plan *p, int S, const void *in, void *out, const ga_epilogue *epi,
+int launch_ring(const ga_plan *p, int S, int G, const void *in, void *out, const ga_epilogue *epi,
                 bool anal, hipStream_t stream) {
-  const int npp = (S + 1) / 2;
-  const int nblk = 8 * ((p->nrings + 7) / 8) * npp;
-  // synthesis needs one length-L work array; analysis also assembles the
-  // full ring (n = 4 r <= 4 nside) there
-  const int lds_elems = anal ? std::max(p->max_L, 4 * p->nside) : p->max_L;
-  const size_t lds = size_t(lds_elems) * 2 * sizeof(T);
-  GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED,
-             "ring transform needs %zu bytes of LDS (nside %d, %zu-byte reals)", lds, p->nside,
-             sizeof(T));
   using tab = typename tab_t<T>::type;
   const tab *blu = reinterpret_cast<const tab *>(sizeof(T) == 8 ? (const void *)p->d_blu64
-                                                                : (const void *)p->d_blu32);
+                                                               : (const void *)p->d_blu32);
   const tab *tw = reinterpret_cast<const tab *>(sizeof(T) == 8 ? (const void *)p->d_tw64
-                                                               : (const void *)p->d_tw32);
-#define GA_RING_LAUNCH(NT_, MAXC_)                                                             \
-  do {                                                                                        \
-    if (anal) {                                                                               \
-      auto kfn = ring_anal_kernel<T, NT_, MAXC_>;                                             \
-      GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn,                                     \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));\
-      hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,         \
-                         p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,        \
-                         p->d_blu_off, blu, tw, p->max_L, (const T *)in, S, npp, p->npix,     \
-                         T(4.0 * M_PI / double(p->npix)), (cplx<T> *)out);                     \
-    } else {                                                                                  \
-      auto kfn = ring_synth_kernel<T, NT_, MAXC_>;                                            \
-      GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn,                                     \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));\
-      hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,         \
-                         p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,        \
-                         p->d_blu_off, blu, tw, p->max_L, (const cplx<T> *)in, S, npp,        \
-                         (T *)out, p->npix, epi);                                             \
-    }                                                                                         \
-  } while (0)
+                                                              : (const void *)p->d_tw32);
   const int max_r = p->max_r;
-  if (max_r <= 256) GA_RING_LAUNCH(256, 1);
-  else if (max_r <= 1024) GA_RING_LAUNCH(256, 4);
-  else if (max_r <= 2048) GA_RING_LAUNCH(256, 8);
-  else if (max_r <= 4096) GA_RING_LAUNCH(512, 8);
-  else GA_REQUIRE(false, GA_ERR_UNSUPPORTED, "ring length 4 * %d not supported", max_r);
-#undef GA_RING_LAUNCH
+  if (anal) {
+    const int npp = (S + 1) / 2;
+    const int nblk = 8 * ((p->nrings + 7) / 8) * npp;
+    // analysis assembles the full ring (n = 4 r <= 4 nside) in LDS as well
+    const size_t lds = size_t(padded_len(std::max(p->max_L, 4 * p->nside))) * 2 * sizeof(T);
+    GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED,
+               "ring analysis needs %zu bytes of LDS (nside %d)", lds, p->nside);
+#define GA_ANAL_LAUNCH(NT_, MAXC_)                                                             \
+  do {                                                                                        \
+    auto kfn = ring_anal_kernel<T, NT_, MAXC_>;                                               \
+    GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn,                                       \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
+    hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,           \
+                       p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,          \
+                       p->d_blu_off, blu, tw, p->max_L, (const T *)in, S, npp, p->npix,       \
+                       T(4.0 * M_PI / double(p->npix)), (cplx<T> *)out);                       \
+  } while (0)
+    if (max_r <= 256) GA_ANAL_LAUNCH(256, 1);
+    else if (max_r <= 1024) GA_ANAL_LAUNCH(256, 4);
+    else if (max_r <= 2048) GA_ANAL_LAUNCH(256, 8);
+    else if (max_r <= 4096) GA_ANAL_LAUNCH(512, 8);
+    else GA_REQUIRE(false, GA_ERR_UNSUPPORTED, "ring length 4 * %d not supported", max_r);
+#undef GA_ANAL_LAUNCH
+    GA_LAUNCH_CHECK();
+    return GA_OK;
+  }
+  // synthesis: P pairs per workgroup when the layout groups shells by 8 and
+  // P work arrays fit in LDS
+  const int ldx = padded_len(p->max_L);
+  int P = 1;
+  // FP32 with shells grouped by 4 or more: 2 pairs per workgroup up to
+  // r = 2048 (register budget), else 1
+  if (sizeof(T) == 4 && G % 4 == 0 && S > 2 && max_r <= 2048 &&
+      size_t(2) * ldx * 2 * sizeof(T) <= 128 * 1024)
+    P = 2;
+  GA_REQUIRE(G == S || G % (2 * P) == 0, GA_ERR_INVALID, "shell group %d incompatible", G);
+  const int nunit = (S + 2 * P - 1) / (2 * P);
+  const int nblk = 8 * ((p->nrings + 7) / 8) * nunit;
+  const size_t lds = size_t(P) * ldx * 2 * sizeof(T);
+  GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds);
+#define GA_SYNTH_LAUNCH(NT_, MAXC_, P_)                                                        \
+  do {                                                                                        \
+    auto kfn = ring_synth_kernel<T, NT_, MAXC_, P_>;                                          \
+    GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn,                                       \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
+    hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,           \
+                       p->d_ring_start, p->d_ring_n, p->d_blu_off, blu, tw, p->max_L, ldx,    \
+                       (const cplx<T> *)in, S, G, nunit, (T *)out, p->npix, epi);             \
+  } while (0)
+  if constexpr (sizeof(T) == 4) {
+    if (P == 2) {
+      if (max_r <= 512) GA_SYNTH_LAUNCH(256, 2, 2);
+      else GA_SYNTH_LAUNCH(512, 4, 2);
+    }
+  }
+  if (P == 1) {
+    if (max_r <= 256) GA_SYNTH_LAUNCH(256, 1, 1);
+    else if (max_r <= 1024) GA_SYNTH_LAUNCH(256, 4, 1);
+    else if (max_r <= 2048) GA_SYNTH_LAUNCH(256, 8, 1);
+    else if (max_r <= 4096) GA_SYNTH_LAUNCH(512, 8, 1);
+    else GA_REQUIRE(false, GA_ERR_UNSUPPORTED, "ring length 4 * %d not supported", max_r);
+  }
+#undef GA_SYNTH_LAUNCH
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
 
 }  // namespace
 
-int launch_ring_synth_f64(const ga_plan *p, int S, const double *F, double *maps,
+int launch_ring_synth_f64(const ga_plan *p, int S, int G, const double *F, double *maps,
                           const ga_epilogue *epi, hipStream_t s) {
-  return launch_ring<double>(p, S, F, maps, epi, false, s);
+  return launch_ring<double>(p, S, G, F, maps, epi, false, s);
 }
-int launch_ring_synth_f32(const ga_plan *p, int S, const float *F, float *maps,
+int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
                           const ga_epilogue *epi, hipStream_t s) {
-  return launch_ring<float>(p, S, F, maps, epi, false, s);
+  return launch_ring<float>(p, S, G, F, maps, epi, false, s);
 }
 int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s) {
-  return launch_ring<double>(p, S, maps, F, nullptr, true, s);
+  return launch_ring<double>(p, S, S, maps, F, nullptr, true, s);
 }
 int launch_ring_anal_f32(const ga_plan *p, int S, const float *maps, float *F, hipStream_t s) {
-  return launch_ring<float>(p, S, maps, F, nullptr, true, s);
+  return launch_ring<float>(p, S, S, maps, F, nullptr, true, s);
 }
 
 }  // namespace ga

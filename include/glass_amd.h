@@ -50,6 +50,15 @@ typedef struct ga_epilogue {
 
 typedef struct ga_plan ga_plan;
 
+/* Process-wide tuning knobs of the FP32-MFMA Legendre stage (set by the
+ * autotuner; no counterpart in the reference, which has no device code). */
+enum ga_tune_key {
+  GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2 or 0 = automatic */
+  GA_TUNE_MFMA_CHUNK = 2  /* quads per accumulation chunk, 0 = no chunking */
+};
+int ga_set_tuning(int32_t key, int32_t value);
+int ga_get_tuning(int32_t key, int32_t *value);
+
 /* Last error message of the calling host thread ("" if none). */
 const char *ga_last_error(void);
 
