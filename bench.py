@@ -32,6 +32,7 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 MFMA_FP32_PEAK = 157.3e12  # MI355X dense FP32 matrix FLOP/s
+MFMA_FP16_PEAK = 2.5e15    # MI355X dense FP16 matrix FLOP/s
 HBM_PEAK = 8.0e12
 
 
@@ -45,6 +46,10 @@ def parse():
     ap.add_argument("--shells", type=int, default=64)
     ap.add_argument("--fp64-maps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="seconds of Poisson arrivals for the serving-loop section (0 = skip)")
+    ap.add_argument("--uneven-maps", type=int, default=128,
+                    help="maps in the uneven-batch section (0 = skip)")
     return ap.parse_args()
 
 
@@ -84,6 +89,64 @@ def cpu_baseline(nside: int, lmax: int, gls, step: int):
                    f"alm {t_alm:.2f} s, synthesis {t_synth:.1f} s (scaled), lognormal "
                    f"{t_ln:.2f} s"),
     }
+
+
+def serving_section(seconds: float):
+    """Serving loop: Poisson arrivals at 5 requests/s (seed 4) for ``seconds``,
+    nside in {256, 512, 1024, 2048, 4096} (weights .3/.3/.2/.15/.05), 10-80
+    shells, lmax = 2 nside, ncorr 3, FP32 MFMA synthesis + lognormal,
+    admission against a 240 GB HBM budget, continuous batching per geometry."""
+    import torch
+
+    from glass_amd import _lib, serving
+
+    reqs = serving.poisson_requests(rate=5.0, duration=seconds, seed=4)
+    ex = serving.DeviceExecutor()
+    for ns in sorted({r.nside for r in reqs}):  # plans are built at server start-up
+        _lib.get_plan(ns, 2 * ns)
+    torch.cuda.synchronize()
+
+    def progress(now, done, total):
+        print(f"[bench] serving t={now:.0f}s completed {done}/{total}", file=sys.stderr, flush=True)
+
+    sched = serving.run(reqs, ex, 240e9, progress=progress)
+    out = serving.summarize(sched)
+    out.update({"arrival_seconds": seconds, "rate_per_s": 5.0, "seed": 4, "budget_gb": 240,
+                "wall_s": sched.wall,
+                "latency_definition": "arrival to all maps of the request resident in HBM"})
+    return out
+
+
+def uneven_section(nmaps: int):
+    """Uneven batch: nmaps maps (seed 3), nside uniform over {256..2048}, lmax =
+    2 nside - 1, per-map tilt, lognormal + shear, captured in one HIP graph."""
+    import torch
+
+    from glass_amd.uneven import UnevenBatch, uneven_specs
+
+    ub = UnevenBatch(uneven_specs(n=nmaps, seed=3), shear=True, tune=True)
+    ub.capture()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        ub.replay()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ub.check()
+    errs = {}
+    for key, eng in ub.engines.items():
+        ref = eng.fp64_reference_map(0)
+        got = eng.maps[0].double()
+        rms = torch.sqrt(torch.mean(ref * ref))
+        errs[f"nside{key[0]}"] = {"max_rel": float(torch.max(torch.abs(got - ref)) / rms),
+                                  "rms_rel": float(torch.sqrt(torch.mean((got - ref) ** 2)) / rms)}
+    t = min(ts)
+    return {"maps": nmaps, "groups": {f"nside{k[0]}": len(v) for k, v in ub.groups.items()},
+            "graph_replay_ms": 1e3 * t, "maps_per_s": nmaps / t, "shear": True,
+            "tuning": {f"nside{k[0]}": {"path": c["path"], "nt": c["nt"]}
+                       for k, c in ub.tuning.items()},
+            "accuracy_vs_fp64_first_map": errs}
 
 
 def main():
@@ -188,6 +251,23 @@ def main():
         "accuracy_vs_fp64": errs,
         "cpu_baseline": None,
     }
+    # the extra sections run after the headline numbers, on freed memory
+    del eng
+    torch.cuda.empty_cache()
+    if args.uneven_maps > 0:
+        print("[bench] uneven batch", file=sys.stderr, flush=True)
+        try:
+            out["uneven_batch"] = uneven_section(args.uneven_maps)
+        except Exception as e:  # the headline line is still printed
+            out["uneven_batch"] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
+    if args.serve_seconds > 0:
+        print("[bench] serving loop", file=sys.stderr, flush=True)
+        try:
+            out["serving"] = serving_section(args.serve_seconds)
+        except Exception as e:
+            out["serving"] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         step = 32 if nside >= 1024 else 1

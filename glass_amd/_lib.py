@@ -30,6 +30,7 @@ GA_ERR_WORKSPACE = 6
 
 TUNE_MFMA_TILES = 1
 TUNE_MFMA_CHUNK = 2
+TUNE_MFMA_PATH = 3
 
 T_NORMAL = 0
 T_LOGNORMAL = 1
@@ -59,7 +60,7 @@ _SIGNATURES = {
     "ga_map2alm_workspace_size": [_vp, _i32, _i32, ctypes.POINTER(_sz)],
     "ga_alm2map_f64": [_vp, _vp, _vp, _vp, _sz, _vp],
     "ga_alm2map_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp],
-    "ga_legendre_synth_batch_f32": [_vp, _i32, _vp, _vp, _vp],
+    "ga_legendre_synth_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_ring_synth_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp],
     "ga_map2alm_f64": [_vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
     "ga_map2alm_batch_f32": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
@@ -70,7 +71,7 @@ _SIGNATURES = {
     "ga_glass_to_healpix_alm_f64": [_i32, _vp, _vp, _vp],
     "ga_iternorm_state_doubles": [_i32, ctypes.POINTER(_i32)],
     "ga_iternorm_batch": [_i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
-    "ga_fields_alm": [_i32, _i32, _i32, _i32, _vp, _vp, _i32, _u64, _vp, _vp, _vp],
+    "ga_fields_alm": [_i32, _i32, _i32, _i32, _vp, _vp, _i32, _u64, _vp, _i32, _vp, _vp],
     "ga_transform_f64": [_i64, _vp, ctypes.POINTER(Epilogue), _vp, _vp],
 }
 

@@ -1,0 +1,102 @@
+"""Autotuner of the batched MFMA Legendre stage.
+
+For one geometry and batch width it times the candidate kernel
+configurations -- the split-FP16 path with one or two 32-column tiles per
+wave, and the FP32 MFMA path -- on the engine's own alm (HIP events on the
+launch stream, median of ``reps``), keeps the fastest, and caches the choice
+per (nside, lmax, shells) in a JSON file (``GLASS_AMD_TUNE_CACHE``, default
+``~/.cache/glass_amd/legendre_tune.json``).  The choice is applied through the
+library's process-wide tuning knobs, which launches read when they are issued
+(also during graph capture).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import pathlib
+
+import numpy as np
+
+from . import _lib
+
+# (path, tiles): path 2 = split FP16, 1 = FP32 MFMA
+CANDIDATES = ((2, 1), (2, 2), (1, 2))
+
+
+def cache_path() -> pathlib.Path:
+    p = os.environ.get("GLASS_AMD_TUNE_CACHE")
+    if p:
+        return pathlib.Path(p)
+    return pathlib.Path.home() / ".cache" / "glass_amd" / "legendre_tune.json"
+
+
+def _load() -> dict:
+    try:
+        return json.loads(cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(d: dict) -> None:
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        tmp = p.with_suffix(".tmp")
+        tmp.write_text(json.dumps(d, indent=1, sort_keys=True))
+        tmp.replace(p)
+    except OSError:
+        pass  # an unwritable cache only costs a re-tune
+
+
+def apply(cfg: dict) -> None:
+    _lib.call("ga_set_tuning", _lib.TUNE_MFMA_PATH, int(cfg["path"]))
+    _lib.call("ga_set_tuning", _lib.TUNE_MFMA_TILES, int(cfg["nt"]))
+    _lib.call("ga_set_tuning", _lib.TUNE_MFMA_CHUNK, int(cfg.get("chunk", 0)))
+
+
+def current() -> dict:
+    out = {}
+    for name, key in (("path", _lib.TUNE_MFMA_PATH), ("nt", _lib.TUNE_MFMA_TILES),
+                      ("chunk", _lib.TUNE_MFMA_CHUNK)):
+        v = _lib.ctypes.c_int32()
+        _lib.call("ga_get_tuning", key, _lib.ctypes.byref(v))
+        out[name] = v.value
+    return out
+
+
+def tune_legendre(eng, candidates=CANDIDATES, reps: int = 3, use_cache: bool = True) -> dict:
+    """Pick the fastest Legendre configuration for a ``LognormalShells``-like
+    engine (attributes nside, lmax, S and methods stage_fields /
+    stage_legendre) and apply it."""
+    T = _lib.torch()
+    key = f"{eng.nside}:{eng.lmax}:{eng.S}"
+    cache = _load() if use_cache else {}
+    if key in cache:
+        apply(cache[key])
+        return cache[key]
+    eng.stage_fields()
+    times = {}
+    for path, nt in candidates:
+        cfg = {"path": path, "nt": nt, "chunk": 0}
+        apply(cfg)
+        eng.stage_legendre()  # warm-up (code load, caches)
+        ts = []
+        for _ in range(reps):
+            a = T.cuda.Event(enable_timing=True)
+            b = T.cuda.Event(enable_timing=True)
+            a.record()
+            eng.stage_legendre()
+            b.record()
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        times[f"{path}/{nt}"] = float(np.median(ts))
+    best = min(times, key=times.get)
+    path, nt = (int(x) for x in best.split("/"))
+    cfg = {"path": path, "nt": nt, "chunk": 0, "ms": times[best], "all_ms": times}
+    apply(cfg)
+    if use_cache:
+        cache = _load()
+        cache[key] = cfg
+        _save(cache)
+    return cfg

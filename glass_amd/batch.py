@@ -44,11 +44,74 @@ def shell_spectra(nshell: int, lmax: int, *, amp=lambda i: 1.0, rho=(0.6, 0.3, 0
     return gls
 
 
+def shear_factor(lmax: int, pixwin_ratio=None) -> np.ndarray:
+    """kappa -> gamma E-mode factor of shear_from_convergence
+    (glass/lensing.py:414-417, and the product of the three almxfl factors of
+    from_convergence, :316-363): -sqrt((l+2)(l+1) l (l-1)) / (l (l+1)), 0 for l < 2,
+    times pw2/pw0 when a pixel-window ratio is given."""
+    ell = np.arange(lmax + 1, dtype=np.float64)
+    f = np.sqrt((ell + 2) * (ell + 1) * ell * np.clip(ell - 1, 0, None))
+    f /= np.clip(ell * (ell + 1), 1, None)
+    f *= -1
+    if pixwin_ratio is not None:
+        f *= np.asarray(pixwin_ratio, dtype=np.float64)[: lmax + 1]
+    return f
+
+
+class ShearShells:
+    """Batched convergence -> shear for S maps of one geometry, device resident:
+    FP32 analysis with Jacobi refinement (map2alm, iter passes), the
+    kappa -> gamma factor (almxfl), spin-2 synthesis with B = 0.  The batched
+    form of shear_from_convergence(kappa, lmax, discretized=False)
+    (glass/lensing.py:374-428); gamma[s] = (gamma1, gamma2) of map s."""
+
+    def __init__(self, nside: int, lmax: int, nshell: int, *, niter: int = 3,
+                 pixwin_ratio=None):
+        T = _lib.torch()
+        _lib.require_gpu()
+        self.nside, self.lmax, self.S, self.niter = int(nside), int(lmax), int(nshell), int(niter)
+        self.npix = 12 * self.nside**2
+        self.nalm = (self.lmax + 1) * (self.lmax + 2) // 2
+        self.plan = _lib.get_plan(self.nside, self.lmax)
+        self.plan.prepare_spin(2)
+        # one workspace serves both steps (the analysis needs ring buffer +
+        # partial sums for S maps, the spin synthesis a ring buffer for 2 S)
+        nbytes = max(self.plan.workspace_bytes(self.S, 4, analysis=True),
+                     self.plan.workspace_bytes(2 * self.S, 4))
+        self.ws = _lib.workspace(nbytes)
+        self.alm = T.empty((self.nalm, self.S), dtype=T.complex64, device="cuda")
+        self.zero = T.zeros_like(self.alm)
+        self.tmp = T.empty((self.S, self.npix), dtype=T.float32, device="cuda")
+        self.fl = T.as_tensor(shear_factor(self.lmax, pixwin_ratio)).to("cuda")
+        self.gamma = T.empty((self.S, 2, self.npix), dtype=T.float32, device="cuda")
+
+    def stage_analysis(self, kappa):
+        if tuple(kappa.shape) != (self.S, self.npix) or not kappa.is_contiguous():
+            raise ValueError("kappa must be a contiguous [nshell, npix] float32 device tensor")
+        _lib.call("ga_map2alm_batch_f32", self.plan.h, self.S, _lib.ptr(kappa),
+                  _lib.ptr(self.alm), self.niter, _lib.ptr(self.tmp), _lib.ptr(self.ws),
+                  self.ws.numel(), _lib.stream_ptr())
+        _lib.call("ga_almxfl", self.lmax, self.S, 4, _lib.ptr(self.alm), _lib.ptr(self.fl),
+                  int(self.fl.numel()), _lib.ptr(self.alm), _lib.stream_ptr())
+
+    def stage_spin(self):
+        _lib.call("ga_alm2map_spin_batch_f32", self.plan.h, 2, self.S, _lib.ptr(self.alm),
+                  _lib.ptr(self.zero), _lib.ptr(self.gamma), _lib.ptr(self.ws), self.ws.numel(),
+                  _lib.stream_ptr())
+
+    def run(self, kappa):
+        self.stage_analysis(kappa)
+        self.stage_spin()
+        return self.gamma
+
+
 class LognormalShells:
-    """S correlated lognormal shells generated in one device batch."""
+    """S correlated lognormal shells generated in one device batch (with
+    ``shear=True`` also the shear of every shell, its map taken as the
+    convergence: ``gamma`` [S][2][npix])."""
 
     def __init__(self, gls, nside: int, *, ncorr: int, shift: float = 1.0, seed: int = 0,
-                 lmax: int | None = None):
+                 lmax: int | None = None, shear: bool = False, shear_lmax: int | None = None):
         T = _lib.torch()
         _lib.require_gpu()
         self.S = _fields.nfields_from_nspectra(len(gls))
@@ -76,6 +139,14 @@ class LognormalShells:
         self.plan = _lib.get_plan(self.nside, self.lmax)
         self.F = _lib.workspace(self.plan.workspace_bytes(self.S, 4))
         self.maps = T.empty((self.S, self.npix), dtype=T.float32, device="cuda")
+        self.shear = None
+        if shear:
+            self.shear = ShearShells(self.nside, self.lmax if shear_lmax is None else shear_lmax,
+                                     self.S)
+
+    @property
+    def gamma(self):
+        return None if self.shear is None else self.shear.gamma
 
     # the stages, separately callable so that each can be timed on the stream
     def stage_fields(self):
@@ -83,20 +154,25 @@ class LognormalShells:
                   _lib.ptr(self.cov), _lib.ptr(self.state), _lib.ptr(self.w),
                   _lib.ptr(self.flag), _lib.stream_ptr())
         _lib.call("ga_fields_alm", self.nl, self.k, 0, self.S, _lib.ptr(self.w), None, 1,
-                  self.seed, _lib.ptr(self.alm), None, _lib.stream_ptr())
+                  self.seed, _lib.ptr(self.alm), 0, None, _lib.stream_ptr())
 
     def stage_legendre(self):
         _lib.call("ga_legendre_synth_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),
-                  _lib.ptr(self.F), _lib.stream_ptr())
+                  _lib.ptr(self.F), self.F.numel(), _lib.stream_ptr())
 
     def stage_rings(self):
         _lib.call("ga_ring_synth_batch_f32", self.plan.h, self.S, _lib.ptr(self.F),
                   _lib.ptr(self.maps), _lib.ptr(self.epi), _lib.stream_ptr())
 
+    def stage_shear(self):
+        if self.shear is not None:
+            self.shear.run(self.maps)
+
     def run(self):
         self.stage_fields()
         self.stage_legendre()
         self.stage_rings()
+        self.stage_shear()
         return self.maps
 
     def check(self) -> None:
@@ -116,7 +192,7 @@ class LognormalShells:
                   _lib.stream_ptr())
         alm = T.empty(self.nalm, dtype=T.complex128, device="cuda")
         _lib.call("ga_fields_alm", self.nl, self.k, shell, 1, _lib.ptr(w[shell]), None, 1,
-                  self.seed, None, _lib.ptr(alm), _lib.stream_ptr())
+                  self.seed, None, 0, _lib.ptr(alm), _lib.stream_ptr())
         x = _hp._synth_f64(alm, self.nside, self.lmax)
         y = T.empty_like(x)
         _lib.call("ga_transform_f64", x.numel(), _lib.ptr(x),

@@ -70,17 +70,18 @@ extern "C" int ga_alm2map_batch_f32(const ga_plan *p, int32_t S, const float *al
   GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE,
              "workspace %zu < %zu bytes", ws_bytes, ring_bytes(p, S, 4));
   hipStream_t s = (hipStream_t)stream;
-  float *F = (float *)ws;
-  int rc = launch_legendre_synth_mfma(p, S, alm, F, s);
+  int rc = launch_legendre_synth_batch(p, S, alm, ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, S, kMfmaGroup, F, maps, epi, s);
+  return launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)ws, maps, epi, s);
 }
 
 extern "C" int ga_legendre_synth_batch_f32(const ga_plan *p, int32_t S, const float *alm,
-                                           float *F, void *stream) {
+                                           void *ws, size_t ws_bytes, void *stream) {
   GA_PLAN_OK(p);
-  GA_REQUIRE(S >= 1 && alm && F, GA_ERR_INVALID, "invalid argument");
-  return launch_legendre_synth_mfma(p, S, alm, F, (hipStream_t)stream);
+  GA_REQUIRE(S >= 1 && alm && ws, GA_ERR_INVALID, "invalid argument");
+  GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
+             ws_bytes, ring_bytes(p, S, 4));
+  return launch_legendre_synth_batch(p, S, alm, ws, (hipStream_t)stream);
 }
 
 extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float *F, float *maps,
@@ -111,7 +112,7 @@ static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter
         if (rc) return rc;
         rc = launch_ring_synth_f64(p, 1, 1, (const double *)F, (double *)map_tmp, nullptr, s);
       } else {
-        rc = launch_legendre_synth_mfma(p, S, (const float *)alm, (float *)F, s);
+        rc = launch_legendre_synth_batch(p, S, (const float *)alm, ws, s);
         if (rc) return rc;
         rc = launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp, nullptr,
                                    s);

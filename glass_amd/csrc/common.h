@@ -47,8 +47,22 @@ void clear_error();
 struct ga_tuning {
   int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto)
   int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
+  int mfma_path = 0;    // Legendre stage: 0 auto (= split FP16), 1 FP32 MFMA, 2 split FP16
 };
 ga_tuning &tuning();
+
+// l rows per slab of the FP16-split MFMA Legendre stage (K = 16 per parity)
+constexpr int kSlab = 32;
+__host__ __device__ inline int64_t nslab_of(int64_t lmax, int64_t m) { return (lmax - m + kSlab) / kSlab; }
+
+// Workspace of the FP32 batched synthesis (sizes for nbatch maps, ncols = 2
+// nbatch real columns): the ring buffer F, then the split-FP16 B operands of
+// the Legendre stage, then one int32 exponent per column.  Each part starts
+// at a multiple of 256 bytes.
+struct f32_ws_layout {
+  size_t f_bytes, b_off, b_bytes, e_off, total;
+};
+f32_ws_layout f32_workspace(const ga_plan *p, int nbatch);
 
 // ---- sizes ----------------------------------------------------------------
 inline int64_t nalm_of(int64_t lmax) { return (lmax + 1) * (lmax + 2) / 2; }
@@ -77,6 +91,10 @@ struct ga_plan {
   // lambda_mm = c_m sin^m theta
   double2 *d_ab = nullptr;
   double *d_cm = nullptr;
+  // FP16 MFMA stage: l-slabs of kSlab rows per m; slab_off[m] = index of the
+  // first slab of m (lmax + 2 entries, slab_off[lmax + 1] = total)
+  int64_t *d_slab_off = nullptr;
+  int64_t nslab = 0;
   // spin recurrence coefficients (A, B, C) per (l, m) for s = 1, 2 (lazily built)
   double *d_spin[3] = {nullptr, nullptr, nullptr};
   // spin seed constants per m: [+s: (signed mantissa, exponent)] then [-s: ...]

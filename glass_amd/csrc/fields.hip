@@ -133,7 +133,7 @@ __global__ void iternorm_kernel(int nl, int kprev0, int k, int kmax, int nshell,
 template <bool HOSTZ>
 __global__ void fields_alm_kernel(int nl, int k, int first_shell, int nshell, int64_t nalm,
                                   const double *__restrict__ w, const double2 *__restrict__ zring,
-                                  int nslot, uint64_t seed, float2 *__restrict__ out32,
+                                  int nslot, uint64_t seed, float2 *__restrict__ out32, int ldo,
                                   double2 *__restrict__ out64) {
   const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= nalm * nshell) return;
@@ -159,7 +159,7 @@ __global__ void fields_alm_kernel(int nl, int k, int first_shell, int nshell, in
     im = fma(wl[j], z.y, im);
   }
   if (m == 0) { re += im; im = 0.0; }
-  if (out32) out32[gid] = float2{float(re), float(im)};
+  if (out32) out32[i * ldo + s] = float2{float(re), float(im)};
   else out64[i] = double2{re, im};
 }
 
@@ -172,7 +172,7 @@ constexpr int kFaBlock = 256, kFaChunk = 16;
 
 __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
     int nl, int k, int first_shell, int nshell, int64_t nalm, const double *__restrict__ w,
-    uint64_t seed, float2 *__restrict__ out32) {
+    uint64_t seed, float2 *__restrict__ out32, int ldo) {
   __shared__ float2 tile[kFaBlock][kFaChunk + 1];
   const int tid = threadIdx.x;
   const int64_t i0 = int64_t(blockIdx.x) * kFaBlock;
@@ -218,10 +218,10 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
       tile[tid][c] = float2{float(re), float(im)};
     }
     __syncthreads();
-    // cooperative store: element (r, c) of the tile -> out[(i0 + r) nshell + s0 + c]
+    // cooperative store: element (r, c) of the tile -> out[(i0 + r) ldo + s0 + c]
     for (int e = tid; e < nrows * cnt; e += kFaBlock) {
       const int r = e / cnt, c = e - r * cnt;
-      out32[(i0 + r) * nshell + s0 + c] = tile[r][c];
+      out32[(i0 + r) * ldo + s0 + c] = tile[r][c];
     }
     __syncthreads();
   }
@@ -303,13 +303,15 @@ extern "C" int ga_iternorm_batch(int32_t nl, int32_t kprev, int32_t k, int32_t k
 
 extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
                              const double *w, const double *zring, int32_t nslot, uint64_t seed,
-                             float *out_f32, double *out_f64, void *stream) {
+                             float *out_f32, int32_t ldo, double *out_f64, void *stream) {
   clear_error();
   GA_REQUIRE(nl > 0 && k >= 0 && first_shell >= 0 && nshell > 0 && w, GA_ERR_INVALID,
              "invalid argument to ga_fields_alm");
   GA_REQUIRE((out_f32 != nullptr) != (out_f64 != nullptr), GA_ERR_INVALID,
              "exactly one of out_f32 / out_f64 must be given");
   GA_REQUIRE(out_f64 == nullptr || nshell == 1, GA_ERR_INVALID, "out_f64 requires nshell == 1");
+  if (ldo == 0) ldo = nshell;
+  GA_REQUIRE(ldo >= nshell, GA_ERR_INVALID, "ldo %d < nshell %d", ldo, nshell);
   GA_REQUIRE(zring == nullptr || nslot >= nshell + k, GA_ERR_INVALID,
              "normal ring buffer needs nslot >= nshell + k (%d < %d)", nslot, nshell + k);
   const int64_t nalm = int64_t(nl) * (nl + 1) / 2;
@@ -318,19 +320,20 @@ extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t
   if (!zring && out_f32 && k + 1 <= kWinMax) {
     hipLaunchKernelGGL(fields_alm_window_kernel, dim3(nblocks(nalm, kFaBlock)), dim3(kFaBlock),
                        0, (hipStream_t)stream, nl, k, first_shell, nshell, nalm, w, seed,
-                       reinterpret_cast<float2 *>(out_f32));
+                       reinterpret_cast<float2 *>(out_f32), ldo);
     GA_LAUNCH_CHECK();
     return GA_OK;
   }
   if (zring)
     hipLaunchKernelGGL(fields_alm_kernel<true>, grid, block, 0, (hipStream_t)stream, nl, k,
                        first_shell, nshell, nalm, w, reinterpret_cast<const double2 *>(zring),
-                       nslot, seed, reinterpret_cast<float2 *>(out_f32),
+                       nslot, seed, reinterpret_cast<float2 *>(out_f32), ldo,
                        reinterpret_cast<double2 *>(out_f64));
   else
     hipLaunchKernelGGL(fields_alm_kernel<false>, grid, block, 0, (hipStream_t)stream, nl, k,
                        first_shell, nshell, nalm, w, nullptr, 1, seed,
-                       reinterpret_cast<float2 *>(out_f32), reinterpret_cast<double2 *>(out_f64));
+                       reinterpret_cast<float2 *>(out_f32), ldo,
+                       reinterpret_cast<double2 *>(out_f64));
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

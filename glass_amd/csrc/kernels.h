@@ -8,8 +8,16 @@ namespace ga {
 // Legendre stage, FP64 plain: alm (healpy order complex128) -> F [ring][m] complex128
 int launch_legendre_synth_f64(const ga_plan *p, const double *alm, double *F, hipStream_t s);
 
-// Legendre stage, FP32 MFMA batched: alm [idx][S] complex64 -> F [ring][m][S] complex64
+// Legendre stage, FP32 MFMA batched: alm [idx][S] complex64 -> F (grouped ring buffer)
 int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float *F, hipStream_t s);
+// Legendre stage, split-FP16 MFMA batched: same in/out; bws / colexp: the B
+// operand and column-exponent parts of the FP32 workspace (f32_workspace)
+int launch_legendre_synth_f16(const ga_plan *p, int S, const float *alm, float *F, void *bws,
+                              int32_t *colexp, hipStream_t s);
+// the batched Legendre stage on the path chosen by the tuning knob; ws is the
+// FP32 workspace with F at its start
+int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
+                                hipStream_t s);
 
 // Ring stage of synthesis: F -> maps [S][npix] with fused transform.  F holds
 // the shells in groups of G: offset ((ring ngrp + s/G)(lmax+1) + m) G + s%G

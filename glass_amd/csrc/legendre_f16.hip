@@ -1,0 +1,407 @@
+// Batched Legendre stage of the scalar synthesis on split-FP16 MFMA (gfx950).
+//
+// Same contraction as the FP32-MFMA stage,
+//     F_m[ring, s] = sum_l lambda_lm(theta_ring) a_lm[s],
+// split by the parity of l - m (north = E + O, south = E - O), but each FP32
+// operand is carried as a pair of FP16 numbers, x = x_hi + x_lo (22 significant
+// bits), and the product is formed from three v_mfma_f32_32x32x16_f16:
+//     lambda a ~ lambda_hi a_hi + lambda_hi a_lo + lambda_lo a_hi
+// (the dropped lambda_lo a_lo is 2^-22 relative).  Every MFMA sums 16 products
+// before adding to the FP32 accumulator, so the accumulator is rounded once per
+// 16 terms instead of once per 2 (FP32 MFMA): the split path is both faster
+// (FP16 MFMA has 16x the FP32 rate) and no less accurate.
+//
+// Ranges: the normalised lambda_lm is bounded by sqrt((2l+1)/4pi) < 2^6 for
+// l < 16384; the recurrence carries lambda 2^8 so that |lambda 2^8| < 2^14
+// fits FP16 and small values keep their low part in the normal range.  The a_lm
+// of each real column are scaled by a power of two that puts the column's
+// maximum in [2^12, 2^13).  The accumulator is unscaled in the epilogue.
+//
+// Work decomposition as in the FP32 stage: a workgroup = 4 waves = 256 ring
+// pairs of one m and NT 32-column tiles; a lane owns one ring pair and runs the
+// FP64 recurrence (scaled seed, no underflow).  The l-range of m is cut into
+// slabs of 32 rows (K = 16 per parity).  Per slab a lane forms its 32 lambda
+// values, splits and packs them as FP16 pairs, and v_permlane32_swap moves the
+// upper K half between the wave halves so that each 32-ring row tile holds the
+// 32 x 16 A operand (lane L: ring L % 32, k = 8 (L / 32) .. + 7).  B comes
+// pre-split and pre-swizzled (one 16-byte load per lane for the high and one
+// for the low part), prepared once per batch by the two small kernels below.
+#include "common.h"
+#include "device.h"
+#include "kernels.h"
+
+namespace ga {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kLamExp = 8;   // lambda is carried as lambda 2^8
+constexpr int kColTop = 13;  // column maxima are scaled below 2^13
+
+__device__ __forceinline__ uint32_t bits_h2(f16x2 h) { return __builtin_bit_cast(uint32_t, h); }
+
+// x = hi + lo for two values at once: hi rounded toward zero (one packed
+// conversion; its error is captured exactly by the residual), lo = the FP32
+// residual rounded to nearest.
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t &hi, uint32_t &lo) {
+  const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
+  const float ra = a - float(h[0]), rb = b - float(h[1]);
+  f16x2 l;
+  l[0] = (_Float16)ra;
+  l[1] = (_Float16)rb;
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = bits_h2(l);
+}
+
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// per-column max |a| as float bit patterns (non-negative floats order like
+// unsigned integers); 64 columns x 4 row streams per block
+__global__ __launch_bounds__(256) void colmax_kernel(int64_t nrows, int ncols,
+                                                     const float *__restrict__ alm,
+                                                     uint32_t *__restrict__ cmax) {
+  __shared__ uint32_t red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < ncols; c0 += 64) {
+    const int c = c0 + tx;
+    uint32_t mx = 0;
+    if (c < ncols)
+      for (int64_t r = int64_t(blockIdx.x) * 4 + ty; r < nrows; r += int64_t(gridDim.x) * 4)
+        mx = max(mx, __float_as_uint(fabsf(alm[r * ncols + c])));
+    red[ty][tx] = mx;
+    __syncthreads();
+    if (ty == 0 && c < ncols) {
+      mx = max(max(red[0][tx], red[1][tx]), max(red[2][tx], red[3][tx]));
+      if (mx) atomicMax(&cmax[c], mx);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int col_shift(uint32_t bits) {
+  const float mx = __uint_as_float(bits);
+  if (!(mx > 0.0f) || !isfinite(mx)) return 0;
+  int e;
+  (void)frexpf(mx, &e);  // mx < 2^e
+  return kColTop - e;
+}
+
+// B operand units: unit = (slab * 2 + parity) * nct + column tile; lane L of a
+// unit holds rows l0 + p + 16 (L / 32) + 2 j (j = 0..7) of column tile * 32 +
+// L % 32, as 8 high halves (first 64 x 16 bytes) and 8 low halves (next).
+__global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int ncols, int nct,
+                                                        int64_t nunit,
+                                                        const int64_t *__restrict__ slab_off,
+                                                        const float *__restrict__ alm,
+                                                        const uint32_t *__restrict__ cmax,
+                                                        uint4 *__restrict__ B,
+                                                        int32_t *__restrict__ colexp) {
+  const int64_t gid = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t unit = gid >> 6;
+  const int L = threadIdx.x & 63;
+  if (unit >= nunit) return;
+  const int64_t sg = unit / (2 * nct);
+  const int rem = int(unit - sg * 2 * nct);
+  const int p = rem / nct, ct = rem - p * nct;
+  // m: slab_off[m] <= sg < slab_off[m + 1]
+  int lo = 0, hi = lmax + 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (slab_off[mid] <= sg) lo = mid;
+    else hi = mid;
+  }
+  const int m = lo;
+  const int l0 = m + kSlab * int(sg - slab_off[m]);
+  const int c = ct * 32 + (L & 31);
+  const int sh = c < ncols ? col_shift(cmax[c]) : 0;
+  if (sg == 0 && p == 0 && L < 32) colexp[c] = sh;
+  const float *__restrict__ col = alm + hp_index(lmax, m, m) * int64_t(ncols) + c;
+  uint32_t hv[4], lv[4];
+#pragma unroll
+  for (int j = 0; j < 8; j += 2) {
+    float v[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int l = l0 + p + 16 * (L >> 5) + 2 * (j + q);
+      v[q] = (c < ncols && l <= lmax) ? ldexpf(col[int64_t(l - m) * ncols], sh) : 0.0f;
+    }
+    split_pair(v[0], v[1], hv[j / 2], lv[j / 2]);
+  }
+  B[unit * 128 + L] = uint4{hv[0], hv[1], hv[2], hv[3]};
+  B[unit * 128 + 64 + L] = uint4{lv[0], lv[1], lv[2], lv[3]};
+}
+
+template <int NT, int MINW>
+__global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
+    int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
+    const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
+    const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
+    const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
+    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F) {
+  // XCD-aware decode: block b runs on XCD b % 8; XCD x takes m = x, x + 8, ...
+  // (largest l-range first), all (ring group, column group) tiles of one m
+  // back to back so they share that m's B slabs in the XCD's L2.
+  const int b = blockIdx.x;
+  const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
+  const int m = 8 * (kq / G) + xcd;
+  if (m > lmax) return;
+  const int rem = kq - (kq / G) * G;
+  const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rp0 = rg * 256 + wave * 64;
+  if (rp0 >= npairs) return;  // wave-uniform; the kernel has no barriers
+  const int rp = rp0 + lane;
+  const int rpc = rp < npairs ? rp : npairs - 1;
+  const double x = z[rpc];
+  double p0, p1 = 0.0;  // lambda_{l-2}, lambda_{l-1} (times 2^8); the table entry at
+  int k;                // l = m is (1, -1), so one step from (seed, 0) yields lambda_mm
+  legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
+
+  const int64_t base = hp_index(lmax, m, m);
+  const double2 *__restrict__ co = ab + base;
+  const int64_t s0 = slab_off[m];
+  const int nsl = int(nslab_of(lmax, m));
+  const int half = lane >> 5, col_in = lane & 31;
+  const int colbase = cg * 32 * NT;
+  // column tiles past the last real one re-read the last tile (never stored)
+  int ctile[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) ctile[ct] = min(cg * NT + ct, nct - 1);
+
+  f32x16 acc[2][2][NT];  // [parity][row tile][col tile]
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
+
+  // recurrence coefficients, one group of 4 rows ahead (their scalar loads
+  // then overlap the previous group's arithmetic)
+  const int lastc = lmax - m;  // last valid row index of this m
+  double2 cn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) cn[j] = co[min(j, lastc)];
+
+  for (int s = 0; s < nsl; ++s) {
+    const int l0 = m + kSlab * s;
+    // B operands of this slab, consumed after the recurrence below
+    uint4 bh[2][NT], bl[2][NT];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const uint4 *__restrict__ u = B + (((s0 + s) * 2 + p) * nct + ctile[ct]) * 128;
+        bh[p][ct] = u[lane];
+        bl[p][ct] = u[64 + lane];
+      }
+
+    const bool chk = wave_any(k < 0);
+    uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const int lg = l0 + 4 * g;
+      double2 c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = cn[j];
+      {
+        const int ln = lg + 4 - m;  // next group's first row index
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cn[j] = co[min(ln + j, lastc)];
+      }
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      if (lg <= lmax) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double pn = c[j].x * (x * p1 - c[j].y * p0);
+          p0 = p1;
+          p1 = pn;
+          v[j] = p1;
+        }
+        if (chk) {
+          // k is fixed within a group: a scaled lane's values are < 2^-290
+          // (zero in FP32); rescale once per group (4 steps grow |p| by far
+          // less than the 2^700 of headroom above the threshold)
+          if (k != 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = 0.0;
+          }
+          if (k < 0 && fabs(p1) > 0x1p300) {
+            p1 *= 0x1p-600;
+            p0 *= 0x1p-600;
+            ++k;
+          }
+        }
+        if (lg + 3 > lmax) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (lg + j > lmax) v[j] = 0.0;
+        }
+      }
+      split_pair(float(v[0]), float(v[2]), ph[0][g], pl[0][g]);
+      split_pair(float(v[1]), float(v[3]), ph[1][g], pl[1][g]);
+    }
+
+    // row tiles with any nonzero lambda in this slab (k only grows, so a lane
+    // still scaled at the end of the slab contributed nothing)
+    const uint64_t on = __ballot(k == 0);
+    const bool t0 = uint32_t(on) != 0u, t1 = uint32_t(on >> 32) != 0u;
+    if (t0 || t1) {
+      // A operands: words 0..3 = groups 0..3 (k 0..7), words 4..7 = k 8..15;
+      // swap(X, Y) gives tile 0 = [own X | partner's Y], tile 1 = [partner's X | own Y]
+      uint32_t ah[2][2][4], al[2][2][4];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const auto sh = __builtin_amdgcn_permlane32_swap(ph[p][w], ph[p][w + 4], false, false);
+          const auto sl = __builtin_amdgcn_permlane32_swap(pl[p][w], pl[p][w + 4], false, false);
+          ah[p][0][w] = sh[0];
+          ah[p][1][w] = sh[1];
+          al[p][0][w] = sl[0];
+          al[p][1][w] = sl[1];
+        }
+      f16x8 Ah[2][2], Al[2][2], Bh[2][NT], Bl[2][NT];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          Ah[p][t] = __builtin_bit_cast(f16x8, uint4{ah[p][t][0], ah[p][t][1], ah[p][t][2], ah[p][t][3]});
+          Al[p][t] = __builtin_bit_cast(f16x8, uint4{al[p][t][0], al[p][t][1], al[p][t][2], al[p][t][3]});
+        }
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          Bh[p][ct] = __builtin_bit_cast(f16x8, bh[p][ct]);
+          Bl[p][ct] = __builtin_bit_cast(f16x8, bl[p][ct]);
+        }
+      }
+      // the three products, accumulators interleaved so that dependent MFMAs
+      // are 4 NT - 1 instructions apart
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            if ((t == 0 && !t0) || (t == 1 && !t1)) continue;
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+              const f16x8 a = (q == 2) ? Al[p][t] : Ah[p][t];
+              const f16x8 bb = (q == 1) ? Bl[p][ct] : Bh[p][ct];
+              acc[p][t][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb, acc[p][t][ct], 0, 0, 0);
+            }
+          }
+    }
+  }
+
+  // epilogue (as in the FP32 stage): D layout of 32x32: lane L, element e ->
+  // row 8 (e/4) + 4 (L/32) + e%4, column L % 32.  Unscale by 2^-(8 + shift of
+  // the column), north = E + O, south = E - O, times the ring phase e^{i m phi0}
+  // (each lane computes its own ring's phase in FP64; rows fetch it by
+  // shuffle; the (re, im) columns of a shell sit in lanes (2c, 2c + 1)).
+  // Ring buffer layout: real column c of ring r at
+  //   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8).
+  float phc, phs;
+  {
+    const int pnum = ring_pnum[rpc], pden = ring_pden[rpc];
+    double ps = 0.0, pc = 1.0;
+    if (pnum) sincospi(2.0 * double((int64_t(m) * pnum) % pden) / double(pden), &ps, &pc);
+    phc = float(pc);
+    phs = float(ps);
+  }
+  float fac[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) {
+    const int col = colbase + ct * 32 + col_in;
+    fac[ct] = col < ncols ? ldexpf(1.0f, -(kLamExp + colexp[col])) : 0.0f;
+  }
+  const int ngrp = (ncols + 7) / 8;
+  const bool is_im = (col_in & 1) != 0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int row = 32 * t + 8 * (e >> 2) + 4 * half + (e & 3);
+      const float c = __shfl(phc, row, 64), sn = __shfl(phs, row, 64);
+      const int rpo = rp0 + row;
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const int col = colbase + ct * 32 + col_in;
+        const float ev = acc[0][t][ct][e] * fac[ct], ov = acc[1][t][ct][e] * fac[ct];
+        const float nv = ev + ov, sv = ev - ov;
+        const float no = __shfl_xor(nv, 1, 64), so = __shfl_xor(sv, 1, 64);
+        // re' = re c - im s,  im' = re s + im c
+        const float nrot = is_im ? (no * sn + nv * c) : (nv * c - no * sn);
+        const float srot = is_im ? (so * sn + sv * c) : (sv * c - so * sn);
+        if (rpo < npairs && col < ncols) {
+          const float nrz = (m == 0 && is_im) ? 0.0f : nrot;
+          const float srz = (m == 0 && is_im) ? 0.0f : srot;
+          const int64_t gofs = (int64_t(col >> 3) * (lmax + 1) + m) * 8 + (col & 7);
+          F[int64_t(rpo) * ngrp * (lmax + 1) * 8 + gofs] = nrz;
+          if (rpo < npairs - 1)
+            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * 8 + gofs] = srz;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int launch_legendre_synth_f16(const ga_plan *p, int S, const float *alm, float *F, void *bws,
+                              int32_t *colexp, hipStream_t stream) {
+  const int ncols = 2 * S;
+  const int nct = (ncols + 31) / 32;
+  uint4 *B = reinterpret_cast<uint4 *>(bws);
+  // per-column maxima scratch, right after the nct * 32 column exponents
+  uint32_t *cmax = reinterpret_cast<uint32_t *>(colexp) + nct * 32;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(unsigned((ncols + 255) / 256)), dim3(256), 0, stream,
+                     cmax, ncols, 0u);
+  GA_LAUNCH_CHECK();
+  const int64_t nrows = p->nalm;
+  int nblk = int(std::min<int64_t>((nrows + 255) / 256, 2048));
+  hipLaunchKernelGGL(colmax_kernel, dim3(unsigned(nblk)), dim3(256), 0, stream, nrows, ncols, alm,
+                     cmax);
+  GA_LAUNCH_CHECK();
+  const int64_t nunit = p->nslab * 2 * nct;
+  hipLaunchKernelGGL(swizzle_b_kernel, dim3(unsigned((nunit * 64 + 255) / 256)), dim3(256), 0,
+                     stream, p->lmax, ncols, nct, nunit, p->d_slab_off, alm, cmax, B, colexp);
+  GA_LAUNCH_CHECK();
+
+  const ga_tuning &tu = tuning();
+  int nt = tu.mfma_nt;
+  if (nt != 1 && nt != 2) nt = ncols <= 32 ? 1 : 2;
+  const int nrg = (p->npairs + 255) / 256;
+  const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
+  const int nm8 = (p->lmax + 1 + 7) / 8;
+  const int64_t nblk2 = int64_t(8) * nm8 * nrg * ncg;
+  GA_REQUIRE(nblk2 < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
+#define GA_F16_LAUNCH(NT_, MW_)                                                                \
+  hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_>), dim3(unsigned(nblk2)), dim3(256), \
+                     0, stream, p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, p->d_z,   \
+                     p->d_sth, p->d_ab, p->d_cm, p->d_ring_pnum, p->d_ring_pden,               \
+                     p->d_slab_off, B, colexp, F)
+  if (nt == 1) GA_F16_LAUNCH(1, 2);
+  else GA_F16_LAUNCH(2, 2);
+#undef GA_F16_LAUNCH
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
+                                hipStream_t stream) {
+  float *F = static_cast<float *>(ws);
+  if (tuning().mfma_path == 1) return launch_legendre_synth_mfma(p, S, alm, F, stream);
+  const f32_ws_layout w = f32_workspace(p, S);
+  char *base = static_cast<char *>(ws);
+  return launch_legendre_synth_f16(p, S, alm, F, base + w.b_off,
+                                   reinterpret_cast<int32_t *>(base + w.e_off), stream);
+}
+
+}  // namespace ga

@@ -189,6 +189,10 @@ extern "C" int ga_set_tuning(int32_t key, int32_t value) {
       GA_REQUIRE(value >= 0 && value <= 4096, GA_ERR_INVALID, "mfma chunk out of range");
       tuning().mfma_chunk = value;
       return GA_OK;
+    case GA_TUNE_MFMA_PATH:
+      GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "mfma path must be 0, 1 or 2");
+      tuning().mfma_path = value;
+      return GA_OK;
     default:
       GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
@@ -200,6 +204,7 @@ extern "C" int ga_get_tuning(int32_t key, int32_t *value) {
   switch (key) {
     case GA_TUNE_MFMA_TILES: *value = tuning().mfma_nt; return GA_OK;
     case GA_TUNE_MFMA_CHUNK: *value = tuning().mfma_chunk; return GA_OK;
+    case GA_TUNE_MFMA_PATH: *value = tuning().mfma_path; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }
@@ -269,6 +274,15 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
       cm[size_t(m)] = (m & 1) ? -v : v;
     }
     if ((rc = upload(&p->d_cm, cm.data(), cm.size() * 8, &p->table_bytes))) return fail(rc);
+    std::vector<int64_t> so(size_t(lmax + 2));
+    int64_t ns = 0;
+    for (int64_t m = 0; m <= lmax; ++m) {
+      so[size_t(m)] = ns;
+      ns += nslab_of(lmax, m);
+    }
+    so[size_t(lmax + 1)] = ns;
+    p->nslab = ns;
+    if ((rc = upload(&p->d_slab_off, so.data(), so.size() * 8, &p->table_bytes))) return fail(rc);
   }
 
   // ---- ring FFT tables: every ring has n = 4 r pixels
@@ -337,7 +351,7 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
 extern "C" int ga_plan_destroy(ga_plan *p) {
   if (!p) return GA_OK;
   void *ptrs[] = {p->d_z, p->d_sth, p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,
-                  p->d_ab, p->d_cm, p->d_spin[0], p->d_spin[1], p->d_spin[2],
+                  p->d_ab, p->d_cm, p->d_slab_off, p->d_spin[0], p->d_spin[1], p->d_spin[2],
                   p->d_spin_seed[0], p->d_spin_seed[1], p->d_spin_seed[2], p->d_blu_off,
                   p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32};
   for (void *q : ptrs)
@@ -357,8 +371,27 @@ extern "C" int ga_workspace_size(const ga_plan *p, int32_t nbatch, int32_t real_
   GA_REQUIRE(p && bytes, GA_ERR_INVALID, "NULL argument");
   GA_REQUIRE(nbatch >= 1, GA_ERR_INVALID, "nbatch must be >= 1");
   GA_REQUIRE(real_bytes == 4 || real_bytes == 8, GA_ERR_INVALID, "real_bytes must be 4 or 8");
-  // fp32 ring buffers hold whole groups of 8 shells (the MFMA stage layout)
-  const int nb = real_bytes == 4 ? (nbatch + 7) / 8 * 8 : nbatch;
-  *bytes = size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nb) * 2 * size_t(real_bytes);
+  if (real_bytes == 4) {
+    *bytes = f32_workspace(p, nbatch).total;
+    return GA_OK;
+  }
+  *bytes = size_t(p->nrings) * size_t(p->lmax + 1) * size_t(nbatch) * 2 * size_t(real_bytes);
   return GA_OK;
 }
+
+namespace ga {
+f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  f32_ws_layout w;
+  // the ring buffer holds whole groups of 8 shells (the MFMA stage layout)
+  const size_t nb = size_t(nbatch + 7) / 8 * 8;
+  w.f_bytes = up(size_t(p->nrings) * size_t(p->lmax + 1) * nb * 8);
+  const size_t nct = size_t(2 * nbatch + 31) / 32;  // 32-column tiles
+  // per (slab, parity, tile): 64 lanes x (8 high + 8 low halves) = 2 KiB
+  w.b_off = w.f_bytes;
+  w.b_bytes = up(size_t(p->nslab) * 2 * nct * 2048);
+  w.e_off = w.b_off + w.b_bytes;
+  w.total = w.e_off + up(nct * 32 * 4 * 2);  // exponents + maxima scratch
+  return w;
+}
+}  // namespace ga

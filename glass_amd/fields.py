@@ -202,12 +202,12 @@ class _GRFEngine:
         if self.precision == "fp64":
             alm = T.empty(self.z_size, dtype=T.complex128, device="cuda")
             _lib.call("ga_fields_alm", self.nl, self.k, self.t, 1, _lib.ptr(w), zptr, self.nslot,
-                      self.seed, None, _lib.ptr(alm), _lib.stream_ptr())
+                      self.seed, None, 0, _lib.ptr(alm), _lib.stream_ptr())
             maps = [_hp._synth_f64(alm, self.nside, self.lmax)]
         else:
             alm = T.empty((self.z_size, B), dtype=T.complex64, device="cuda")
             _lib.call("ga_fields_alm", self.nl, self.k, self.t, B, _lib.ptr(w), zptr, self.nslot,
-                      self.seed, _lib.ptr(alm), None, _lib.stream_ptr())
+                      self.seed, _lib.ptr(alm), 0, None, _lib.stream_ptr())
             epi = (_lib.Epilogue * B)()
             for j in range(B):
                 tr = transforms[j] or (_lib.T_NORMAL, 1.0, 0.0)

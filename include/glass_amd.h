@@ -50,11 +50,12 @@ typedef struct ga_epilogue {
 
 typedef struct ga_plan ga_plan;
 
-/* Process-wide tuning knobs of the FP32-MFMA Legendre stage (set by the
+/* Process-wide tuning knobs of the batched MFMA Legendre stage (set by the
  * autotuner; no counterpart in the reference, which has no device code). */
 enum ga_tune_key {
   GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2 or 0 = automatic */
-  GA_TUNE_MFMA_CHUNK = 2  /* quads per accumulation chunk, 0 = no chunking */
+  GA_TUNE_MFMA_CHUNK = 2, /* FP32 path: quads per accumulation chunk, 0 = none */
+  GA_TUNE_MFMA_PATH = 3   /* 0 = automatic (split FP16), 1 = FP32 MFMA, 2 = split FP16 */
 };
 int ga_set_tuning(int32_t key, int32_t value);
 int ga_get_tuning(int32_t key, int32_t *value);
@@ -102,11 +103,14 @@ int ga_alm2map_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
                          size_t ws_bytes, void *stream);
 
 /* The two stages of ga_alm2map_batch_f32, callable separately (for timing
- * and for callers that keep the ring buffer): the FP32-MFMA Legendre stage
- * alm [idx][S] -> ring buffer F [nrings][lmax+1][S], and the ring-FFT stage
- * F -> maps [S][npix] with the fused transform. */
+ * and for callers that keep the ring buffer): the MFMA Legendre stage
+ * alm [idx][S] -> ring buffer F, which occupies the start of the workspace
+ * (ga_workspace_size(plan, S, 4) bytes; shells in groups of 4, complex
+ * element (ring, m, s) at ((ring ceil(S/4) + s/4)(lmax+1) + m) 4 + s%4), and
+ * the ring-FFT stage F (= the workspace pointer) -> maps [S][npix] with the
+ * fused transform. */
 int ga_legendre_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
-                                float *F, void *stream);
+                                void *workspace, size_t ws_bytes, void *stream);
 int ga_ring_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *F, float *maps,
                             const ga_epilogue *epi, void *stream);
 
@@ -172,11 +176,13 @@ int ga_iternorm_batch(int32_t nl, int32_t kprev, int32_t k, int32_t kmax, int32_
  *     GLASS-order complex128 vectors, z_t in slot t % nslot (nslot >=
  *     nshell + k so that every shell of the call sees its window); or
  *   - counter-based Philox4x32-10 (seed, shell, index) when zring == NULL.
- * Output: batch layout [idx][nshell] complex64 (out_f32) or, for nshell == 1,
- * healpy-order complex128 (out_f64); exactly one of the two is non-NULL. */
+ * Output: batch layout [idx][ldo] complex64 (out_f32; shell s of the call in
+ * column s, ldo >= nshell, 0 = nshell, so that several calls can fill the
+ * column ranges of one shared batch) or, for nshell == 1, healpy-order
+ * complex128 (out_f64); exactly one of the two is non-NULL. */
 int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
                   const double *w, const double *zring, int32_t nslot, uint64_t seed,
-                  float *out_f32, double *out_f64, void *stream);
+                  float *out_f32, int32_t ldo, double *out_f64, void *stream);
 
 /* Pointwise transforms on a float64 map (n pixels), out may equal x.
  * glass/grf/_transformations.py. */

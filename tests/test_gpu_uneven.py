@@ -1,0 +1,97 @@
+"""Uneven batch under one HIP graph, and the Legendre autotuner.
+
+Every graph replay (with eager work and synchronisations between replays) must
+reproduce the eager run exactly (same kernels, same inputs); each map is
+within 1e-5 of the FP64 path (max |error| / map RMS).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from glass_amd import autotune  # noqa: E402
+from glass_amd.batch import LognormalShells, shell_spectra  # noqa: E402
+from glass_amd.uneven import UnevenBatch, independent_gls, map_spectrum, uneven_specs  # noqa: E402
+
+
+@pytest.fixture(autouse=True)
+def _tune_cache(tmp_path, monkeypatch):
+    monkeypatch.setenv("GLASS_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    saved = None
+    try:
+        saved = autotune.current()
+    except Exception:
+        pass
+    yield
+    if saved is not None:
+        autotune.apply(saved)
+
+
+def rel(a, b):
+    a, b = a.double(), b.double()
+    return float(torch.max(torch.abs(a - b)) / torch.sqrt(torch.mean(b * b)))
+
+
+def test_uneven_specs_workload():
+    sp = uneven_specs()
+    assert len(sp) == 128
+    ns = np.array([s.nside for s in sp])
+    assert set(ns) <= {256, 512, 1024, 2048} and len(set(ns)) == 4
+    assert all(s.lmax == 2 * s.nside - 1 for s in sp)
+    t = np.array([s.tilt for s in sp])
+    assert t.min() >= -1.5 and t.max() <= -1.0
+    assert uneven_specs() == sp
+
+
+def test_graph_replays_match_eager_and_fp64():
+    specs = uneven_specs(n=12, seed=3, nsides=(16, 32, 64))
+    ub = UnevenBatch(specs, shear=True, tune=True)
+    ub.run_eager()
+    torch.cuda.synchronize()
+    ub.check()
+    ref = {s.index: tuple(x.clone() for x in ub.maps_of(s.index)) for s in specs}
+    ub.capture()
+    for rep in range(3):
+        # eager work between replays overwrites the buffers
+        for eng in ub.engines.values():
+            eng.maps.zero_()
+            eng.gamma.zero_()
+        torch.cuda.synchronize()
+        ub.replay()
+        torch.cuda.synchronize()
+        for s in specs:
+            m, g = ub.maps_of(s.index)
+            assert torch.equal(m, ref[s.index][0]), (rep, s.index)
+            assert torch.equal(g, ref[s.index][1]), (rep, s.index)
+    # accuracy of a few maps against the FP64 path of their engine
+    for key, eng in ub.engines.items():
+        for j in (0, eng.S - 1):
+            assert rel(eng.maps[j], eng.fp64_reference_map(j)) < 1e-5, (key, j)
+    # the per-map spectra: engine j of a group uses map_spectrum(lmax, tilt)
+    key = ub.keys[0]
+    sps = ub.groups[key]
+    gls = independent_gls([map_spectrum(key[1], s.tilt) for s in sps])
+    assert len(gls) == len(sps) * (len(sps) + 1) // 2
+
+
+def test_autotuner_picks_and_caches_a_candidate():
+    S, nside, lmax = 8, 32, 63
+    eng = LognormalShells(shell_spectra(S, lmax), nside, ncorr=3, seed=2, lmax=lmax)
+    cfg = autotune.tune_legendre(eng, reps=2)
+    assert (cfg["path"], cfg["nt"]) in autotune.CANDIDATES
+    assert set(cfg["all_ms"]) == {f"{p}/{n}" for p, n in autotune.CANDIDATES}
+    assert autotune.current()["path"] == cfg["path"]
+    again = autotune.tune_legendre(eng)
+    assert (again["path"], again["nt"]) == (cfg["path"], cfg["nt"])
+    # every candidate gives the same maps to FP32 accuracy
+    outs = []
+    for p, n in autotune.CANDIDATES:
+        autotune.apply({"path": p, "nt": n})
+        eng.run()
+        torch.cuda.synchronize()
+        outs.append(eng.maps.clone())
+    for o in outs[1:]:
+        for j in range(S):
+            assert rel(o[j], outs[0][j]) < 1e-5

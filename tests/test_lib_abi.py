@@ -43,7 +43,7 @@ def test_host_side_errors_without_gpu():
     n = ctypes.c_int32()
     assert L.ga_iternorm_state_doubles(3, ctypes.byref(n)) == 0 and n.value == 16
     assert L.ga_iternorm_state_doubles(-1, ctypes.byref(n)) == _lib.GA_ERR_INVALID
-    rc = L.ga_fields_alm(4, 1, 0, 1, None, None, 0, 0, None, None, None)
+    rc = L.ga_fields_alm(4, 1, 0, 1, None, None, 0, 0, None, 0, None, None)
     assert rc == _lib.GA_ERR_INVALID
     assert b"ga_fields_alm" in L.ga_last_error()
     with pytest.raises(ValueError, match="ga_plan_create"):
