@@ -7,13 +7,16 @@ across shells), normals from the device Philox generator.  One step = one
 batch of S shells through the whole path, inputs resident in HBM:
 
     iternorm -> correlated alm (+ healpy reorder, m = 0 fix)
-    -> Legendre stage on FP32 MFMA (FP64 recurrence) -> ring FFT + lognormal.
+    -> Legendre stage on MFMA (operand preparation, then the contraction with
+       the recurrence in FP64; split-FP16 products by default)
+    -> ring FFT + lognormal.
 
 value = maps/s over the timed steps.  Also reported: the plain FP64 GPU path's
 maps/s on the same configuration, the FP32 maps' error against the FP64 path
-(same normals), the roofline of the Legendre kernel (achieved dense FLOP/s
-from HIP events on the launch stream), per-step p50/p99 latency, and the FP64
-CPU oracle timed on a bounded sample on the host cores.
+(same normals), the roofline of the Legendre contraction kernel (algorithmic
+dense FLOP/s from HIP events on the launch stream), per-step p50/p99 latency,
+an uneven batch under one HIP graph, the serving loop's p50/p99 request
+latency, and the FP64 CPU oracle timed on a bounded sample on the host cores.
 """
 
 from __future__ import annotations
@@ -50,13 +53,17 @@ def parse():
                     help="seconds of Poisson arrivals for the serving-loop section (0 = skip)")
     ap.add_argument("--uneven-maps", type=int, default=128,
                     help="maps in the uneven-batch section (0 = skip)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (a known-size "
+                         "kernel that calibrates hardware byte counters)")
     return ap.parse_args()
 
 
 def read_traffic(nside: int, lmax: int, shells: int):
-    """HBM bytes per launch of the Legendre kernel from the committed rocprof
-    PMC summary (a line 'traffic_bytes_per_launch: <n>' for this workload)."""
-    p = ROOT / "profiles" / f"legendre_mfma_nside{nside}_lmax{lmax}_S{shells}_pmc.md"
+    """HBM bytes per launch of the Legendre contraction kernel from the
+    committed rocprof PMC summary of this workload (its line
+    'traffic_bytes_per_launch: <n>': FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    p = ROOT / "profiles" / f"pmc_lognormal_nside{nside}_lmax{lmax}_S{shells}.md"
     if not p.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
@@ -149,6 +156,16 @@ def uneven_section(nmaps: int):
             "accuracy_vs_fp64_first_map": errs}
 
 
+def _tuning_path() -> int:
+    import ctypes
+
+    from glass_amd import _lib
+
+    v = ctypes.c_int32()
+    _lib.call("ga_get_tuning", _lib.TUNE_MFMA_PATH, ctypes.byref(v))
+    return 1 if v.value == 1 else 2
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -169,6 +186,12 @@ def main():
                         rho=(0.6, 0.3, 0.1), scale=1500.0)
     print(f"[bench] building batch: nside={nside} lmax={lmax} shells={S}", file=sys.stderr,
           flush=True)
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     eng = LognormalShells(gls, nside, ncorr=3, shift=1.0, seed=1234, lmax=lmax)
     for _ in range(args.warmup):
         eng.run()
@@ -176,7 +199,9 @@ def main():
     eng.check()
     print("[bench] warmup done", file=sys.stderr, flush=True)
 
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    # HIP events on the stream the library launches on (torch's current stream,
+    # which every stage passes to the C ABI)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -184,18 +209,23 @@ def main():
         e[0].record()
         eng.stage_fields()
         e[1].record()
-        eng.stage_legendre()
+        eng.stage_operands()
         e[2].record()
-        eng.stage_rings()
+        eng.stage_contract()
         e[3].record()
+        eng.stage_rings()
+        e[4].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     eng.check()
     wall = t1 - t0
-    step_ms = [ev[k][0].elapsed_time(ev[k][3]) for k in range(args.steps)]
-    fields_ms = float(np.mean([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)]))
-    leg_ms = float(np.mean([ev[k][1].elapsed_time(ev[k][2]) for k in range(args.steps)]))
-    ring_ms = float(np.mean([ev[k][2].elapsed_time(ev[k][3]) for k in range(args.steps)]))
+
+    def span(a, b):
+        return float(np.mean([ev[k][a].elapsed_time(ev[k][b]) for k in range(args.steps)]))
+
+    step_ms = [ev[k][0].elapsed_time(ev[k][4]) for k in range(args.steps)]
+    fields_ms, prep_ms, leg_ms, ring_ms = span(0, 1), span(1, 2), span(2, 3), span(3, 4)
+    path = _tuning_path()
     value = args.steps * S / wall
     print(f"[bench] timed {args.steps} steps: {wall:.3f} s", file=sys.stderr, flush=True)
 
@@ -223,6 +253,13 @@ def main():
     flop = 4.0 * nalm * (2 * nside) * S  # dense Legendre contraction, ring pairs
     achieved = flop / (leg_ms * 1e-3)
     traffic = read_traffic(nside, lmax, S)
+    if path == 1:
+        peak, kname, peak_note = MFMA_FP32_PEAK, "legendre_synth_mfma_kernel", "FP32 MFMA dense"
+    else:
+        # three FP16 products (hi*hi + hi*lo + lo*hi) per FP32-accurate product
+        peak, kname = MFMA_FP16_PEAK / 3.0, "legendre_synth_f16_kernel"
+        peak_note = ("FP16 MFMA dense 2.5 PFLOP/s / 3: the split-FP16 contraction issues "
+                     "three FP16 products per algorithmic FP32 product")
     out = {
         "metric": "lognormal maps/s at nside 2048 (FP32-MFMA vs FP64 plain, same GPU); p99 latency",
         "value": value,
@@ -238,12 +275,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched lognormal shells", "nside": nside, "lmax": lmax,
                    "shells_per_batch": S, "ncorr": 3, "rng": "device Philox4x32-10",
-                   "stages": "iternorm + alm + FP32-MFMA Legendre + ring FFT + lognormal"},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_FP32_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / MFMA_FP32_PEAK, "traffic": traffic,
-                     "kernel": "legendre_synth_mfma", "flop_per_launch": flop,
-                     "avg_launch_ms": leg_ms},
-        "stages_ms": {"fields": fields_ms, "legendre_mfma": leg_ms, "ring_fft_lognormal": ring_ms},
+                   "stages": "iternorm + alm + MFMA Legendre (FP64 recurrence) + ring FFT "
+                             "+ lognormal",
+                   "legendre_path": "FP32 MFMA" if path == 1 else "split-FP16 MFMA"},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                     "kernel": kname, "flop_per_launch": flop, "avg_launch_ms": leg_ms,
+                     "peak_note": peak_note},
+        "stages_ms": {"fields": fields_ms, "legendre_operands": prep_ms, "legendre_mfma": leg_ms,
+                      "ring_fft_lognormal": ring_ms},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99)), "unit": "per batch"},
         "fp64_plain_maps_per_s": fp64,

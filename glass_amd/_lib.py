@@ -31,6 +31,7 @@ GA_ERR_WORKSPACE = 6
 TUNE_MFMA_TILES = 1
 TUNE_MFMA_CHUNK = 2
 TUNE_MFMA_PATH = 3
+TUNE_RING_PATH = 4
 
 T_NORMAL = 0
 T_LOGNORMAL = 1
@@ -61,6 +62,8 @@ _SIGNATURES = {
     "ga_alm2map_f64": [_vp, _vp, _vp, _vp, _sz, _vp],
     "ga_alm2map_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp],
     "ga_legendre_synth_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
+    "ga_legendre_prepare_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
+    "ga_legendre_contract_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_ring_synth_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp],
     "ga_map2alm_f64": [_vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
     "ga_map2alm_batch_f32": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _sz, _vp],

@@ -157,7 +157,15 @@ class LognormalShells:
                   self.seed, _lib.ptr(self.alm), 0, None, _lib.stream_ptr())
 
     def stage_legendre(self):
-        _lib.call("ga_legendre_synth_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),
+        self.stage_operands()
+        self.stage_contract()
+
+    def stage_operands(self):
+        _lib.call("ga_legendre_prepare_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),
+                  _lib.ptr(self.F), self.F.numel(), _lib.stream_ptr())
+
+    def stage_contract(self):
+        _lib.call("ga_legendre_contract_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),
                   _lib.ptr(self.F), self.F.numel(), _lib.stream_ptr())
 
     def stage_rings(self):

@@ -48,6 +48,7 @@ struct ga_tuning {
   int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto)
   int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
   int mfma_path = 0;    // Legendre stage: 0 auto (= split FP16), 1 FP32 MFMA, 2 split FP16
+  int ring_path = 1;    // FP32 ring synthesis: 0 auto (= fused passes), 1 radix-4 stages, 2 fused passes
 };
 ga_tuning &tuning();
 
@@ -90,6 +91,7 @@ struct ga_plan {
   // lambda_l = a (x lambda_{l-1} - b lambda_{l-2}); seed constants c_m with
   // lambda_mm = c_m sin^m theta
   double2 *d_ab = nullptr;
+  double2 *d_aab = nullptr;  // (a, a b) per (l, m): the same recurrence, one FMA deep
   double *d_cm = nullptr;
   // FP16 MFMA stage: l-slabs of kSlab rows per m; slab_off[m] = index of the
   // first slab of m (lmax + 2 entries, slab_off[lmax + 1] = total)
@@ -106,6 +108,9 @@ struct ga_plan {
   int64_t *d_blu_off = nullptr;  // per r (0..max_r): offset into chirp tables, -1 if pow2
   double2 *d_blu64 = nullptr;
   float2 *d_blu32 = nullptr;
+  // rings by FFT class (power-of-two r / Bluestein), with the class maxima
+  int32_t *d_rings_pow2 = nullptr, *d_rings_blu = nullptr;
+  int32_t n_pow2 = 0, n_blu = 0, max_r_pow2 = 0, max_r_blu = 0, max_L_blu = 0;
   double2 *d_tw64 = nullptr;  // exp(2 pi i j / max_L), j < max_L
   float2 *d_tw32 = nullptr;
   size_t table_bytes = 0;

@@ -51,18 +51,19 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
 }
 
 // standard complex normal pair (re, im) for GLASS-order index g of shell t:
-// Box-Muller on two 53-bit uniforms built from the four Philox words
+// Box-Muller on two Philox words, u1 = (w0 + 1/2) 2^-32 in (0, 1] (tail to
+// sqrt(-2 ln 2^-33) = 6.76 sigma) and u2 = (w1 >> 8) 2^-24 in [0, 1), with the
+// hardware log2 / sqrt / sin(2 pi x) / cos(2 pi x) in FP32 (relative error a
+// few 1e-7: far below what the statistics of the field can resolve), widened
+// to FP64 so that every path (FP32 batch, FP64 reference) sees the same values
 __device__ __forceinline__ double2 device_normal_pair(uint64_t seed, int64_t t, int64_t g) {
   u32x4 c{{uint32_t(g), uint32_t(uint64_t(g) >> 32), uint32_t(t), uint32_t(uint64_t(t) >> 32)}};
   const u32x4 r = philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
-  const uint64_t a = (uint64_t(r.v[0]) << 21) ^ (uint64_t(r.v[1]) >> 11);  // 53 bits
-  const uint64_t b = (uint64_t(r.v[2]) << 21) ^ (uint64_t(r.v[3]) >> 11);
-  const double u1 = (double(a & ((uint64_t(1) << 53) - 1)) + 0.5) * 0x1p-53;  // (0, 1)
-  const double u2 = double(b & ((uint64_t(1) << 53) - 1)) * 0x1p-53;
-  const double rad = sqrt(-2.0 * log(u1));
-  double s, co;
-  sincospi(2.0 * u2, &s, &co);
-  return double2{rad * co, rad * s};
+  const float u1 = (float(r.v[0]) + 0.5f) * 0x1p-32f;
+  const float u2 = float(r.v[1] >> 8) * 0x1p-24f;
+  const float rad = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(u1));
+  const float sn = __builtin_amdgcn_sinf(u2), cs = __builtin_amdgcn_cosf(u2);
+  return double2{double(rad * cs), double(rad * sn)};
 }
 
 // State per mode l (capacity kmax): M [kmax][kmax] row-major, a [kmax], s,

@@ -43,16 +43,30 @@ constexpr int kColTop = 13;  // column maxima are scaled below 2^13
 
 __device__ __forceinline__ uint32_t bits_h2(f16x2 h) { return __builtin_bit_cast(uint32_t, h); }
 
+// a - (the low / high FP16 half of h, widened): one v_fma_mix_f32 (-h * 1 + a)
+// with the half selected by op_sel; exact, since a - h is representable
+__device__ __forceinline__ float sub_half_lo(float a, uint32_t h) {
+  float r;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
+  return r;
+}
+__device__ __forceinline__ float sub_half_hi(float a, uint32_t h) {
+  float r;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h),
+      "v"(a));
+  return r;
+}
+
 // x = hi + lo for two values at once: hi rounded toward zero (one packed
 // conversion; its error is captured exactly by the residual), lo = the FP32
 // residual rounded to nearest.
 __device__ __forceinline__ void split_pair(float a, float b, uint32_t &hi, uint32_t &lo) {
   const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
-  const float ra = a - float(h[0]), rb = b - float(h[1]);
+  hi = __builtin_bit_cast(uint32_t, h);
+  const float ra = sub_half_lo(a, hi), rb = sub_half_hi(b, hi);
   f16x2 l;
   l[0] = (_Float16)ra;
   l[1] = (_Float16)rb;
-  hi = __builtin_bit_cast(uint32_t, h);
   lo = bits_h2(l);
 }
 
@@ -84,6 +98,38 @@ __global__ __launch_bounds__(256) void colmax_kernel(int64_t nrows, int ncols,
   }
 }
 
+// the same for ncols % 4 == 0 and ncols <= 1024: each thread reads float4s of
+// four fixed columns (T4 = ncols / 4 threads per row, 256 / T4 rows per pass)
+__global__ __launch_bounds__(256) void colmax4_kernel(int64_t nrows, int ncols,
+                                                      const float4 *__restrict__ alm,
+                                                      uint32_t *__restrict__ cmax) {
+  __shared__ uint32_t red[4][256];
+  const int T4 = ncols >> 2, R = 256 / T4;
+  const int tid = threadIdx.x;
+  const int r = tid / T4, q = tid - r * T4;
+  uint32_t mx[4] = {0u, 0u, 0u, 0u};
+  if (r < R) {
+    for (int64_t row = int64_t(blockIdx.x) * R + r; row < nrows; row += int64_t(gridDim.x) * R) {
+      const float4 v = alm[row * T4 + q];
+      mx[0] = max(mx[0], __float_as_uint(fabsf(v.x)));
+      mx[1] = max(mx[1], __float_as_uint(fabsf(v.y)));
+      mx[2] = max(mx[2], __float_as_uint(fabsf(v.z)));
+      mx[3] = max(mx[3], __float_as_uint(fabsf(v.w)));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) red[j][tid] = mx[j];
+  __syncthreads();
+  if (tid < T4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      uint32_t m = red[j][tid];
+      for (int rr = 1; rr < R; ++rr) m = max(m, red[j][rr * T4 + tid]);
+      if (m) atomicMax(&cmax[4 * tid + j], m);
+    }
+  }
+}
+
 __device__ __forceinline__ int col_shift(uint32_t bits) {
   const float mx = __uint_as_float(bits);
   if (!(mx > 0.0f) || !isfinite(mx)) return 0;
@@ -95,29 +141,23 @@ __device__ __forceinline__ int col_shift(uint32_t bits) {
 // B operand units: unit = (slab * 2 + parity) * nct + column tile; lane L of a
 // unit holds rows l0 + p + 16 (L / 32) + 2 j (j = 0..7) of column tile * 32 +
 // L % 32, as 8 high halves (first 64 x 16 bytes) and 8 low halves (next).
+// grid: (ceil(units of m = 0 / 4), lmax + 1): blockIdx.y = m, 4 units per block.
 __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int ncols, int nct,
-                                                        int64_t nunit,
                                                         const int64_t *__restrict__ slab_off,
                                                         const float *__restrict__ alm,
                                                         const uint32_t *__restrict__ cmax,
                                                         uint4 *__restrict__ B,
                                                         int32_t *__restrict__ colexp) {
-  const int64_t gid = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  const int64_t unit = gid >> 6;
+  const int m = blockIdx.y;
+  const int u = blockIdx.x * 4 + (threadIdx.x >> 6);  // unit within m
   const int L = threadIdx.x & 63;
-  if (unit >= nunit) return;
-  const int64_t sg = unit / (2 * nct);
-  const int rem = int(unit - sg * 2 * nct);
+  const int per_slab = 2 * nct;
+  if (u >= int(nslab_of(lmax, m)) * per_slab) return;
+  const int sl = u / per_slab, rem = u - sl * per_slab;
   const int p = rem / nct, ct = rem - p * nct;
-  // m: slab_off[m] <= sg < slab_off[m + 1]
-  int lo = 0, hi = lmax + 1;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (slab_off[mid] <= sg) lo = mid;
-    else hi = mid;
-  }
-  const int m = lo;
-  const int l0 = m + kSlab * int(sg - slab_off[m]);
+  const int64_t sg = slab_off[m] + sl;
+  const int64_t unit = sg * per_slab + rem;
+  const int l0 = m + kSlab * sl;
   const int c = ct * 32 + (L & 31);
   const int sh = c < ncols ? col_shift(cmax[c]) : 0;
   if (sg == 0 && p == 0 && L < 32) colexp[c] = sh;
@@ -159,6 +199,9 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   const int rp = rp0 + lane;
   const int rpc = rp < npairs ? rp : npairs - 1;
   const double x = z[rpc];
+  // the recurrence table holds (a, a b): lambda_l = (a x) lambda_{l-1} - (a b) lambda_{l-2},
+  // so a step has one FMA on its dependency chain (a x and (a b) lambda_{l-2}
+  // are ready a step early)
   double p0, p1 = 0.0;  // lambda_{l-2}, lambda_{l-1} (times 2^8); the table entry at
   int k;                // l = m is (1, -1), so one step from (seed, 0) yields lambda_mm
   legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
@@ -182,15 +225,20 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
 
-  // recurrence coefficients, one group of 4 rows ahead (their scalar loads
-  // then overlap the previous group's arithmetic)
+  // Recurrence coefficients go through LDS, one slab (32 double2) per wave
+  // and buffer: the next slab's are loaded from global memory (lanes 0..31,
+  // one pair each) while the current slab runs, then written to the other
+  // buffer; every step reads its pair with a broadcast LDS load.  Only this
+  // wave touches its buffers, so no workgroup barrier is needed.
+  __shared__ double2 cbuf[4][2][kSlab];
   const int lastc = lmax - m;  // last valid row index of this m
-  double2 cn[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) cn[j] = co[min(j, lastc)];
+  double2 cpre = co[min(lane & (kSlab - 1), lastc)];
+  if (lane < kSlab) cbuf[wave][0][lane] = cpre;
+  __builtin_amdgcn_wave_barrier();
 
   for (int s = 0; s < nsl; ++s) {
     const int l0 = m + kSlab * s;
+    const int cur = s & 1;
     // B operands of this slab, consumed after the recurrence below
     uint4 bh[2][NT], bl[2][NT];
 #pragma unroll
@@ -201,33 +249,50 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
         bh[p][ct] = u[lane];
         bl[p][ct] = u[64 + lane];
       }
+    // next slab's coefficients (clamped past the end; never used there)
+    cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
+    const double2 *__restrict__ cb = cbuf[wave][cur];
 
-    const bool chk = wave_any(k < 0);
     uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
+    const bool slow = (l0 + kSlab - 1 > lmax) || wave_any(k < 0);
+    if (!slow) {
+      // every lane in range and a full slab: the plain recurrence
 #pragma unroll
-    for (int g = 0; g < 8; ++g) {
-      const int lg = l0 + 4 * g;
-      double2 c[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) c[j] = cn[j];
-      {
-        const int ln = lg + 4 - m;  // next group's first row index
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cn[j] = co[min(ln + j, lastc)];
-      }
-      double v[4] = {0.0, 0.0, 0.0, 0.0};
-      if (lg <= lmax) {
+      for (int g = 0; g < 8; ++g) {
+        float v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const double pn = c[j].x * (x * p1 - c[j].y * p0);
+          const double2 c = cb[4 * g + j];
+          const double pn = fma(c.x * x, p1, -(c.y * p0));
           p0 = p1;
           p1 = pn;
-          v[j] = p1;
+          v[j] = float(pn);
         }
-        if (chk) {
+        split_pair(v[0], v[2], ph[0][g], pl[0][g]);
+        split_pair(v[1], v[3], ph[1][g], pl[1][g]);
+        // keep the coefficient loads of later groups from being hoisted here
+        // (they would hold 4 VGPRs each for the whole slab)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        const int lg = l0 + 4 * g;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        bool dead = true;  // every lane still scaled: the group's values are all zero
+        if (lg <= lmax) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const double2 c = cb[4 * g + j];
+            const double pn = fma(c.x * x, p1, -(c.y * p0));
+            p0 = p1;
+            p1 = pn;
+            v[j] = p1;
+          }
           // k is fixed within a group: a scaled lane's values are < 2^-290
           // (zero in FP32); rescale once per group (4 steps grow |p| by far
           // less than the 2^700 of headroom above the threshold)
+          dead = wave_all(k < 0);
           if (k != 0) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) v[j] = 0.0;
@@ -237,22 +302,27 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
             p0 *= 0x1p-600;
             ++k;
           }
-        }
-        if (lg + 3 > lmax) {
+          if (lg + 3 > lmax) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (lg + j > lmax) v[j] = 0.0;
+            for (int j = 0; j < 4; ++j)
+              if (lg + j > lmax) v[j] = 0.0;
+          }
         }
+        if (dead) {
+          ph[0][g] = ph[1][g] = pl[0][g] = pl[1][g] = 0u;
+        } else {
+          split_pair(float(v[0]), float(v[2]), ph[0][g], pl[0][g]);
+          split_pair(float(v[1]), float(v[3]), ph[1][g], pl[1][g]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      split_pair(float(v[0]), float(v[2]), ph[0][g], pl[0][g]);
-      split_pair(float(v[1]), float(v[3]), ph[1][g], pl[1][g]);
     }
+    if (lane < kSlab) cbuf[wave][cur ^ 1][lane] = cpre;
+    __builtin_amdgcn_wave_barrier();
 
-    // row tiles with any nonzero lambda in this slab (k only grows, so a lane
-    // still scaled at the end of the slab contributed nothing)
-    const uint64_t on = __ballot(k == 0);
-    const bool t0 = uint32_t(on) != 0u, t1 = uint32_t(on >> 32) != 0u;
-    if (t0 || t1) {
+    // MFMAs only if some lane has a nonzero lambda in this slab (k only
+    // grows, so a lane still scaled at the end of the slab contributed nothing)
+    if (wave_any(k == 0)) {
       // A operands: words 0..3 = groups 0..3 (k 0..7), words 4..7 = k 8..15;
       // swap(X, Y) gives tile 0 = [own X | partner's Y], tile 1 = [partner's X | own Y]
       uint32_t ah[2][2][4], al[2][2][4];
@@ -282,21 +352,20 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
         }
       }
       // the three products, accumulators interleaved so that dependent MFMAs
-      // are 4 NT - 1 instructions apart
+      // are 4 NT - 1 instructions apart; both row tiles whenever either is on
+      // (straight-line code; the matrix pipe has slack next to the recurrence)
 #pragma unroll
       for (int q = 0; q < 3; ++q)
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int t = 0; t < 2; ++t) {
-            if ((t == 0 && !t0) || (t == 1 && !t1)) continue;
+          for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int ct = 0; ct < NT; ++ct) {
               const f16x8 a = (q == 2) ? Al[p][t] : Ah[p][t];
               const f16x8 bb = (q == 1) ? Bl[p][ct] : Bh[p][ct];
               acc[p][t][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb, acc[p][t][ct], 0, 0, 0);
             }
-          }
     }
   }
 
@@ -354,8 +423,8 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
 
 }  // namespace
 
-int launch_legendre_synth_f16(const ga_plan *p, int S, const float *alm, float *F, void *bws,
-                              int32_t *colexp, hipStream_t stream) {
+int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
+                                int32_t *colexp, hipStream_t stream) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   uint4 *B = reinterpret_cast<uint4 *>(bws);
@@ -366,14 +435,27 @@ int launch_legendre_synth_f16(const ga_plan *p, int S, const float *alm, float *
   GA_LAUNCH_CHECK();
   const int64_t nrows = p->nalm;
   int nblk = int(std::min<int64_t>((nrows + 255) / 256, 2048));
-  hipLaunchKernelGGL(colmax_kernel, dim3(unsigned(nblk)), dim3(256), 0, stream, nrows, ncols, alm,
-                     cmax);
+  if (ncols % 4 == 0 && ncols <= 1024 && (reinterpret_cast<uintptr_t>(alm) & 15) == 0)
+    hipLaunchKernelGGL(colmax4_kernel, dim3(unsigned(nblk)), dim3(256), 0, stream, nrows, ncols,
+                       reinterpret_cast<const float4 *>(alm), cmax);
+  else
+    hipLaunchKernelGGL(colmax_kernel, dim3(unsigned(nblk)), dim3(256), 0, stream, nrows, ncols,
+                       alm, cmax);
   GA_LAUNCH_CHECK();
-  const int64_t nunit = p->nslab * 2 * nct;
-  hipLaunchKernelGGL(swizzle_b_kernel, dim3(unsigned((nunit * 64 + 255) / 256)), dim3(256), 0,
-                     stream, p->lmax, ncols, nct, nunit, p->d_slab_off, alm, cmax, B, colexp);
+  GA_REQUIRE(p->lmax < 65535, GA_ERR_UNSUPPORTED, "lmax too large for the operand preparation");
+  const int units0 = int(nslab_of(p->lmax, 0)) * 2 * nct;
+  hipLaunchKernelGGL(swizzle_b_kernel, dim3(unsigned((units0 + 3) / 4), unsigned(p->lmax + 1)),
+                     dim3(256), 0, stream, p->lmax, ncols, nct, p->d_slab_off, alm, cmax, B,
+                     colexp);
   GA_LAUNCH_CHECK();
+  return GA_OK;
+}
 
+int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
+                                 const int32_t *colexp, hipStream_t stream) {
+  const int ncols = 2 * S;
+  const int nct = (ncols + 31) / 32;
+  const uint4 *B = reinterpret_cast<const uint4 *>(bws);
   const ga_tuning &tu = tuning();
   int nt = tu.mfma_nt;
   if (nt != 1 && nt != 2) nt = ncols <= 32 ? 1 : 2;
@@ -385,7 +467,7 @@ int launch_legendre_synth_f16(const ga_plan *p, int S, const float *alm, float *
 #define GA_F16_LAUNCH(NT_, MW_)                                                                \
   hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_>), dim3(unsigned(nblk2)), dim3(256), \
                      0, stream, p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, p->d_z,   \
-                     p->d_sth, p->d_ab, p->d_cm, p->d_ring_pnum, p->d_ring_pden,               \
+                     p->d_sth, p->d_aab, p->d_cm, p->d_ring_pnum, p->d_ring_pden,               \
                      p->d_slab_off, B, colexp, F)
   if (nt == 1) GA_F16_LAUNCH(1, 2);
   else GA_F16_LAUNCH(2, 2);
@@ -394,14 +476,30 @@ int launch_legendre_synth_f16(const ga_plan *p, int S, const float *alm, float *
   return GA_OK;
 }
 
-int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                hipStream_t stream) {
+int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, void *ws,
+                                  hipStream_t stream) {
+  if (tuning().mfma_path == 1) return GA_OK;  // the FP32 path reads the alm directly
+  const f32_ws_layout w = f32_workspace(p, S);
+  char *base = static_cast<char *>(ws);
+  return launch_legendre_prepare_f16(p, S, alm, base + w.b_off,
+                                     reinterpret_cast<int32_t *>(base + w.e_off), stream);
+}
+
+int launch_legendre_contract_batch(const ga_plan *p, int S, const float *alm, void *ws,
+                                   hipStream_t stream) {
   float *F = static_cast<float *>(ws);
   if (tuning().mfma_path == 1) return launch_legendre_synth_mfma(p, S, alm, F, stream);
   const f32_ws_layout w = f32_workspace(p, S);
-  char *base = static_cast<char *>(ws);
-  return launch_legendre_synth_f16(p, S, alm, F, base + w.b_off,
-                                   reinterpret_cast<int32_t *>(base + w.e_off), stream);
+  const char *base = static_cast<const char *>(ws);
+  return launch_legendre_contract_f16(p, S, F, base + w.b_off,
+                                      reinterpret_cast<const int32_t *>(base + w.e_off), stream);
+}
+
+int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
+                                hipStream_t stream) {
+  int rc = launch_legendre_prepare_batch(p, S, alm, ws, stream);
+  if (rc) return rc;
+  return launch_legendre_contract_batch(p, S, alm, ws, stream);
 }
 
 }  // namespace ga

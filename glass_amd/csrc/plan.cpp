@@ -193,6 +193,10 @@ extern "C" int ga_set_tuning(int32_t key, int32_t value) {
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "mfma path must be 0, 1 or 2");
       tuning().mfma_path = value;
       return GA_OK;
+    case GA_TUNE_RING_PATH:
+      GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "ring path must be 0, 1 or 2");
+      tuning().ring_path = value;
+      return GA_OK;
     default:
       GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
@@ -205,6 +209,7 @@ extern "C" int ga_get_tuning(int32_t key, int32_t *value) {
     case GA_TUNE_MFMA_TILES: *value = tuning().mfma_nt; return GA_OK;
     case GA_TUNE_MFMA_CHUNK: *value = tuning().mfma_chunk; return GA_OK;
     case GA_TUNE_MFMA_PATH: *value = tuning().mfma_path; return GA_OK;
+    case GA_TUNE_RING_PATH: *value = tuning().ring_path; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }
@@ -265,6 +270,10 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
       }
     }
     if ((rc = upload(&p->d_ab, ab.data(), ab.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
+    // the same recurrence as (a, a b): lambda_l = (a x) lambda_{l-1} - (a b) lambda_{l-2},
+    // one FMA on the dependency chain per step
+    for (double2 &v : ab) v.y *= v.x;
+    if ((rc = upload(&p->d_aab, ab.data(), ab.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
     // c_m = (-1)^m sqrt((2m+1)/(4 pi) prod_{k<=m} (2k-1)/(2k)): lambda_mm = c_m sin^m
     std::vector<double> cm(size_t(lmax + 1));
     long double acc = 1.0L;
@@ -339,6 +348,24 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
       tw32[size_t(j)] = float2{float(tw[size_t(j)].x), float(tw[size_t(j)].y)};
     }
     if ((rc = upload(&p->d_blu_off, off.data(), off.size() * 8, &p->table_bytes))) return fail(rc);
+    std::vector<int32_t> lp, lb;
+    for (int32_t ring = 0; ring < p->nrings; ++ring) {
+      const int32_t r = rn[size_t(ring)] / 4;
+      if (is_pow2(r)) {
+        lp.push_back(ring);
+        p->max_r_pow2 = std::max(p->max_r_pow2, r);
+      } else {
+        lb.push_back(ring);
+        p->max_r_blu = std::max(p->max_r_blu, r);
+        p->max_L_blu = std::max<int32_t>(p->max_L_blu, int32_t(blu_len(r)));
+      }
+    }
+    p->n_pow2 = int32_t(lp.size());
+    p->n_blu = int32_t(lb.size());
+    if (!lp.empty() && (rc = upload(&p->d_rings_pow2, lp.data(), lp.size() * 4, &p->table_bytes)))
+      return fail(rc);
+    if (!lb.empty() && (rc = upload(&p->d_rings_blu, lb.data(), lb.size() * 4, &p->table_bytes)))
+      return fail(rc);
     if ((rc = upload(&p->d_blu64, blu.data(), blu.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
     if ((rc = upload(&p->d_blu32, blu32.data(), blu32.size() * sizeof(float2), &p->table_bytes))) return fail(rc);
     if ((rc = upload(&p->d_tw64, tw.data(), tw.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
@@ -351,9 +378,10 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
 extern "C" int ga_plan_destroy(ga_plan *p) {
   if (!p) return GA_OK;
   void *ptrs[] = {p->d_z, p->d_sth, p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,
-                  p->d_ab, p->d_cm, p->d_slab_off, p->d_spin[0], p->d_spin[1], p->d_spin[2],
+                  p->d_ab, p->d_aab, p->d_cm, p->d_slab_off, p->d_spin[0], p->d_spin[1], p->d_spin[2],
                   p->d_spin_seed[0], p->d_spin_seed[1], p->d_spin_seed[2], p->d_blu_off,
-                  p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32};
+                  p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32, p->d_rings_pow2,
+                  p->d_rings_blu};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete p;

@@ -24,6 +24,8 @@
 //
 // The values H are held in registers (each thread owns indices j1 = t + i NT),
 // LDS holds only one length-L work array.
+#include <type_traits>
+
 #include "common.h"
 #include "device.h"
 #include "kernels.h"
@@ -460,6 +462,315 @@ __global__ __launch_bounds__(NT) void ring_anal_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 synthesis, second form: fused radix-2^K passes.
+//
+// After the radix-4 split (as above), the four length-r inverse DFTs of a
+// complex transform run together on four LDS arrays.  Each pass fuses K <= 4
+// radix-2 stages: a thread loads the 2^K elements of one group into
+// registers, runs the K stages there (the twiddles of a group are one table
+// value per stage times constant rotations by 2^(K-s)-th roots of unity),
+// and writes them back to the same places, so a 4096-point transform takes 3
+// passes and 3 barriers (instead of 6 LDS stages), and a thread never holds
+// more than one group.  Power-of-two r: inputs stored bit-reversed, inverse
+// decimation in time -> natural order.  Other r (Bluestein): forward
+// decimation in frequency (natural -> bit-reversed), product with the
+// bit-reversed chirp spectrum, inverse decimation in time.  The two ring
+// classes are launched separately (ring lists), each with LDS for its own
+// transform length.
+
+// e^{2 pi i k / 16} for k = 0..15: cos and sin
+__device__ __forceinline__ float c16(int k) {
+  switch (k & 15) {
+    case 0: return 1.0f;
+    case 1: case 15: return 0.92387953251128674f;
+    case 2: case 14: return 0.70710678118654752f;
+    case 3: case 13: return 0.38268343236508977f;
+    case 4: case 12: return 0.0f;
+    case 5: case 11: return -0.38268343236508977f;
+    case 6: case 10: return -0.70710678118654752f;
+    case 7: case 9: return -0.92387953251128674f;
+    default: return -1.0f;
+  }
+}
+__device__ __forceinline__ float s16(int k) { return c16(k - 4); }
+
+// v e^{SIGN 2 pi i k / 16} (k a compile-time constant after unrolling)
+template <int SIGN>
+__device__ __forceinline__ cplx<float> rot16(cplx<float> v, int k) {
+  k &= 15;
+  if (k == 0) return v;
+  if (k == 4) return SIGN > 0 ? cplx<float>{-v.im, v.re} : cplx<float>{v.im, -v.re};
+  if (k == 8) return cplx<float>{-v.re, -v.im};
+  if (k == 12) return SIGN > 0 ? cplx<float>{v.im, -v.re} : cplx<float>{-v.im, v.re};
+  const float c = c16(k), sn = SIGN * s16(k);
+  return cplx<float>{v.re * c - v.im * sn, v.re * sn + v.im * c};
+}
+
+__device__ __forceinline__ cplx<float> twid(const float2 *__restrict__ tw, int idx, int sign) {
+  const float2 w = tw[idx];
+  return cplx<float>{w.x, sign * w.y};
+}
+
+// One pass fusing K radix-2 stages with half-spans h0, 2 h0, .., 2^(K-1) h0
+// (DIT: stages in increasing span; DIF: decreasing) on A arrays of length N.
+// Group = elements base + q h0 (q < 2^K), base = blk 2^K h0 + b, b < h0.
+template <int K, int NT, bool DIT, int SIGN>
+__device__ __forceinline__ void fused_pass(cplx<float> *X, int ldx, int A, int N, int h0,
+                                           const float2 *__restrict__ tw, int maxL) {
+  constexpr int R = 1 << K;
+  const int lh0 = ilog2(h0);
+  const int lg = ilog2(N) - K;  // groups per array = N / R
+  const int total = A << lg;
+  for (int gg = threadIdx.x; gg < total; gg += NT) {
+    const int a = gg >> lg, g = gg & ((1 << lg) - 1);
+    const int b = g & (h0 - 1), blk = g >> lh0;
+    cplx<float> *Y = X + a * ldx;
+    const int base = (blk << (K + lh0)) + b;
+    cplx<float> v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = Y[pad(base + (q << lh0))];
+    // ws[s] = e^{SIGN 2 pi i b / (2^(s+1) h0)}: one table value per stage
+    cplx<float> ws[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) ws[s] = twid(tw, (b * (maxL >> (s + 1 + lh0))), SIGN);
+#pragma unroll
+    for (int ss = 0; ss < K; ++ss) {
+      const int s = DIT ? ss : K - 1 - ss;
+      const int d = 1 << s;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (q & d) continue;
+        // twiddle of this pair: ws[s] e^{SIGN 2 pi i (q mod d) / (2 d)}
+        const cplx<float> t = rot16<SIGN>(ws[s], (q & (d - 1)) * (16 >> (s + 1)));
+        if (DIT) {
+          const cplx<float> y = cmul(v[q + d], t);
+          const cplx<float> x = v[q];
+          v[q] = cadd(x, y);
+          v[q + d] = csub(x, y);
+        } else {
+          const cplx<float> x = v[q], y = v[q + d];
+          v[q] = cadd(x, y);
+          v[q + d] = cmul(csub(x, y), t);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) Y[pad(base + (q << lh0))] = v[q];
+  }
+  __syncthreads();
+}
+
+// full power-of-two transform of A arrays: DIT (bit-reversed in, natural out)
+// or DIF (natural in, bit-reversed out)
+template <int NT, bool DIT, int SIGN>
+__device__ void fused_fft(cplx<float> *X, int ldx, int A, int N, const float2 *__restrict__ tw,
+                          int maxL) {
+  const int lN = ilog2(N);
+  if (DIT) {
+    for (int l = 0; l < lN;) {
+      const int K = lN - l >= 4 ? 4 : lN - l;
+      switch (K) {
+        case 4: fused_pass<4, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
+        case 3: fused_pass<3, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
+        case 2: fused_pass<2, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
+        default: fused_pass<1, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
+      }
+      l += K;
+    }
+  } else {
+    for (int l = lN; l > 0;) {  // stages with half-spans 2^(l-1) down to 1
+      const int K = l >= 4 ? 4 : l;
+      const int h0 = 1 << (l - K);
+      switch (K) {
+        case 4: fused_pass<4, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
+        case 3: fused_pass<3, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
+        case 2: fused_pass<2, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
+        default: fused_pass<1, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
+      }
+      l -= K;
+    }
+  }
+}
+
+// grid: 8 * ceil(nlist / 8) * nunit blocks (XCD-aware as decode_ring_block),
+// one unit = one complex transform (two shells); rings taken from ring_list;
+// BLU selects the Bluestein form.  All four sub-transforms run together on
+// four LDS arrays (one set of twiddles and barriers per pass).
+template <int NT, int MAXC, bool BLU>
+__global__ __launch_bounds__(NT) void ring_synth2_kernel(
+    int lmax, const int32_t *__restrict__ ring_list, int nlist, const int64_t *__restrict__ ring_start,
+    const int32_t *__restrict__ ring_n, const int64_t *__restrict__ blu_off,
+    const float2 *__restrict__ blu, const float2 *__restrict__ tw, int maxL, int ldx,
+    const cplx<float> *__restrict__ F, int S, int G, int nunit, float *__restrict__ maps,
+    int64_t npix, const ga_epilogue *__restrict__ epi) {
+  extern __shared__ char smem_raw[];
+  cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
+  int li, unit;
+  if (!decode_ring_block(nlist, nunit, li, unit)) return;
+  const int ring = ring_list[li];
+  const int t = threadIdx.x;
+  const int sbase = 2 * unit;
+  const int n = ring_n[ring], r = n >> 2;
+  const int ngrp = (S + G - 1) / G;
+  const cplx<float> *Fu = F + (int64_t(ring) * ngrp + sbase / G) * (lmax + 1) * G + sbase % G;
+  const bool two = sbase + 1 < S;
+  const int64_t boff = BLU ? blu_off[r] : 0;
+  const float2 *__restrict__ chirp = blu + boff;
+  const float2 *__restrict__ bspec = chirp + r;
+  const int L = BLU ? blu_len(r) : r;
+  const int lbits = ilog2(L);
+
+  // ---- fold, radix-4 split (thread owns k1 = t + i NT), sub-transform inputs -> LDS
+#pragma unroll
+  for (int i = 0; i < MAXC; ++i) {
+    const int k1 = t + i * NT;
+    if (k1 < r) {
+      cplx<float> h[4];
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) {
+        cplx<float> g0 = {0.0f, 0.0f}, g1 = {0.0f, 0.0f};
+        const int k = k1 + r * k2;
+        for (int m = k; m <= lmax; m += n) {
+          const cplx<float> *f = Fu + int64_t(m) * G;
+          g0 = cadd(g0, f[0]);
+          if (two) g1 = cadd(g1, f[1]);
+        }
+        for (int m = (k == 0 ? n : n - k); m <= lmax; m += n) {
+          const cplx<float> *f = Fu + int64_t(m) * G;
+          g0 = cadd(g0, cconj(f[0]));
+          if (two) g1 = cadd(g1, cconj(f[1]));
+        }
+        h[k2] = cplx<float>{g0.re - g1.im, g0.im + g1.re};
+      }
+      const cplx<float> w1 = expi_q(k1, n, 0.0f);
+      const cplx<float> w2 = cmul(w1, w1);
+      const cplx<float> w3 = cmul(w2, w1);
+      const cplx<float> s02 = cadd(h[0], h[2]), d02 = csub(h[0], h[2]);
+      const cplx<float> s13 = cadd(h[1], h[3]), d13 = cmuli(csub(h[1], h[3]));
+      cplx<float> y[4] = {cadd(s02, s13), cmul(cadd(d02, d13), w1), cmul(csub(s02, s13), w2),
+                          cmul(csub(d02, d13), w3)};
+      if (BLU) {
+        const float2 c = chirp[k1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = cmul(y[q], cplx<float>{c.x, c.y});
+      }
+      // power-of-two r: inputs in bit-reversed order for the DIT passes
+      const int pos = BLU ? k1 : (lbits ? int(__brev(unsigned(k1)) >> (32 - lbits)) : 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[q * ldx + pad(pos)] = y[q];
+    }
+  }
+  if (BLU) {
+    for (int e = t; e < (L - r) * 4; e += NT) {
+      const int a = e / (L - r), j = r + e - a * (L - r);
+      X[a * ldx + pad(j)] = cplx<float>{0.0f, 0.0f};
+    }
+  }
+  __syncthreads();
+
+  // ---- the four length-r inverse DFTs (Bluestein: forward, chirp spectrum, inverse)
+  if (BLU) {
+    fused_fft<NT, false, -1>(X, ldx, 4, L, tw, maxL);  // natural -> bit-reversed
+    for (int e = t; e < L * 4; e += NT) {
+      const int a = e >> lbits, j = e & (L - 1);
+      const float2 b = bspec[j];  // stored bit-reversed
+      X[a * ldx + pad(j)] = cmul(X[a * ldx + pad(j)], cplx<float>{b.x, b.y});
+    }
+    __syncthreads();
+  }
+  fused_fft<NT, true, +1>(X, ldx, 4, L, tw, maxL);  // bit-reversed -> natural
+
+  // ---- epilogue: pixels 4 j1 .. 4 j1 + 3 of each shell, one 16-byte store
+  const int64_t st = ring_start[ring];
+  int k0 = GA_T_NORMAL, kk1 = GA_T_NORMAL;
+  float l0 = 1.0f, a0 = 0.0f, l1 = 1.0f, a1 = 0.0f;
+  if (epi) {
+    k0 = epi[sbase].kind;
+    l0 = float(epi[sbase].lamda);
+    a0 = float(epi[sbase].param);
+    if (two) {
+      kk1 = epi[sbase + 1].kind;
+      l1 = float(epi[sbase + 1].lamda);
+      a1 = float(epi[sbase + 1].param);
+    }
+  }
+  float4 *out0 = reinterpret_cast<float4 *>(maps + int64_t(sbase) * npix + st);
+  float4 *out1 = reinterpret_cast<float4 *>(maps + int64_t(sbase + 1) * npix + st);
+  for (int j1 = t; j1 < r; j1 += NT) {
+    cplx<float> y[4];
+    cplx<float> cj = {1.0f, 0.0f};
+    if (BLU) {
+      const float2 c = chirp[j1];
+      cj = cplx<float>{c.x, c.y};
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      y[q] = X[q * ldx + pad(j1)];
+      if (BLU) y[q] = cmul(y[q], cj);
+    }
+    out0[j1] = float4{apply_epi<float>(y[0].re, k0, l0, a0), apply_epi<float>(y[1].re, k0, l0, a0),
+                      apply_epi<float>(y[2].re, k0, l0, a0), apply_epi<float>(y[3].re, k0, l0, a0)};
+    if (two)
+      out1[j1] = float4{apply_epi<float>(y[0].im, kk1, l1, a1), apply_epi<float>(y[1].im, kk1, l1, a1),
+                        apply_epi<float>(y[2].im, kk1, l1, a1), apply_epi<float>(y[3].im, kk1, l1, a1)};
+  }
+}
+
+template <int NT, int MAXC, bool BLU>
+int launch_synth2(const ga_plan *p, const int32_t *list, int nlist, int N, int S, int G,
+                  const float *F, float *maps, const ga_epilogue *epi, hipStream_t stream) {
+  if (nlist == 0) return GA_OK;
+  const int ldx = padded_len(N);
+  const size_t lds = size_t(4) * ldx * sizeof(cplx<float>);
+  GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds);
+  const int nunit = (S + 1) / 2;
+  const int nblk = 8 * ((nlist + 7) / 8) * nunit;
+  auto kfn = ring_synth2_kernel<NT, MAXC, BLU>;
+  GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(lds)));
+  hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, list, nlist,
+                     p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32, p->d_tw32,
+                     p->max_L, ldx, reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps,
+                     p->npix, epi);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+// both ring classes with the fused-pass kernel; GA_ERR_UNSUPPORTED (before any
+// launch) if a class exceeds the instantiated sizes, and the caller then uses
+// the first form
+int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
+                           const ga_epilogue *epi, hipStream_t stream) {
+  // a unit's two shells must be adjacent in the ring buffer, and the pixel
+  // stores are 16-byte vectors
+  if (!(G == S || G % 2 == 0)) return GA_ERR_UNSUPPORTED;
+  if (reinterpret_cast<uintptr_t>(maps) & 15) return GA_ERR_UNSUPPORTED;
+  const int Np = p->max_r_pow2, Nb = p->max_L_blu, rb = p->max_r_blu;
+  if (p->n_pow2 > 0 && Np > 4096) return GA_ERR_UNSUPPORTED;
+  if (p->n_blu > 0 && Nb > 4096) return GA_ERR_UNSUPPORTED;
+  int rc = GA_OK;
+  if (p->n_pow2 > 0) {
+    const int32_t *l = p->d_rings_pow2;
+    const int n = p->n_pow2;
+    if (Np <= 256) rc = launch_synth2<256, 1, false>(p, l, n, Np, S, G, F, maps, epi, stream);
+    else if (Np <= 512) rc = launch_synth2<256, 2, false>(p, l, n, Np, S, G, F, maps, epi, stream);
+    else if (Np <= 1024) rc = launch_synth2<256, 4, false>(p, l, n, Np, S, G, F, maps, epi, stream);
+    else if (Np <= 2048) rc = launch_synth2<256, 8, false>(p, l, n, Np, S, G, F, maps, epi, stream);
+    else rc = launch_synth2<512, 8, false>(p, l, n, Np, S, G, F, maps, epi, stream);
+    if (rc) return rc;
+  }
+  if (p->n_blu > 0) {
+    const int32_t *l = p->d_rings_blu;
+    const int n = p->n_blu;
+    if (rb <= 256) rc = launch_synth2<256, 1, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+    else if (rb <= 512) rc = launch_synth2<256, 2, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+    else if (rb <= 1024) rc = launch_synth2<256, 4, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+    else rc = launch_synth2<256, 8, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+  }
+  return rc;
+}
+
 template <typename T>
 int launch_ring(const ga_plan *p, int S, int G, const void *in, void *out, const ga_epilogue *epi,
                 bool anal, hipStream_t stream) {
@@ -544,6 +855,12 @@ int launch_ring_synth_f64(const ga_plan *p, int S, int G, const double *F, doubl
 }
 int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
                           const ga_epilogue *epi, hipStream_t s) {
+  if (tuning().ring_path != 1) {
+    // the second form needs whole units of 2P shells in one group
+    const int rc = launch_ring_synth2_f32(p, S, G, F, maps, epi, s);
+    if (rc != GA_ERR_UNSUPPORTED) return rc;
+    clear_error();
+  }
   return launch_ring<float>(p, S, G, F, maps, epi, false, s);
 }
 int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s) {

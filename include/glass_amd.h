@@ -55,7 +55,9 @@ typedef struct ga_plan ga_plan;
 enum ga_tune_key {
   GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2 or 0 = automatic */
   GA_TUNE_MFMA_CHUNK = 2, /* FP32 path: quads per accumulation chunk, 0 = none */
-  GA_TUNE_MFMA_PATH = 3   /* 0 = automatic (split FP16), 1 = FP32 MFMA, 2 = split FP16 */
+  GA_TUNE_MFMA_PATH = 3,  /* 0 = automatic (split FP16), 1 = FP32 MFMA, 2 = split FP16 */
+  GA_TUNE_RING_PATH = 4   /* FP32 ring synthesis: 1 = radix-4 LDS stages (default),
+                             2 = fused radix-2^K passes, 0 = automatic (= 2) */
 };
 int ga_set_tuning(int32_t key, int32_t value);
 int ga_get_tuning(int32_t key, int32_t *value);
@@ -111,6 +113,15 @@ int ga_alm2map_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
  * fused transform. */
 int ga_legendre_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
                                 void *workspace, size_t ws_bytes, void *stream);
+/* ga_legendre_synth_batch_f32 in its two launches: operand preparation (on
+ * the split-FP16 path: per-column scaling, FP16 splitting and swizzling of the
+ * alm into the workspace; nothing on the FP32 path) and the contraction into
+ * F.  The contraction reads what the preparation of the same nshell and alm
+ * left in the workspace, under the same tuning. */
+int ga_legendre_prepare_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
+                                  void *workspace, size_t ws_bytes, void *stream);
+int ga_legendre_contract_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
+                                   void *workspace, size_t ws_bytes, void *stream);
 int ga_ring_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *F, float *maps,
                             const ga_epilogue *epi, void *stream);
 

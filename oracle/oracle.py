@@ -40,6 +40,10 @@ Pinning (what checks each function, all in ``tests/test_oracle.py``):
 * generate       -- the reference's own invariants (test_fields.py:362-438):
                     same seed -> same first field for ncorr None vs 1; two
                     identical gls -> field 1 equals field 0.
+* philox4x32_10  -- the published known-answer vectors of Philox4x32-10
+                    (Salmon et al., SC'11, Random123 distribution).  The device
+                    generator of this build (not the reference's numpy stream)
+                    is then checked against device_normals built on it.
 
 Parity unpinned: healpy's pixel-weight files (use_pixel_weights=True) and pixwin
 tables are not available, so those inputs are taken from the caller.
@@ -408,3 +412,45 @@ def kappa_to_shear_factor(lmax: int, pw_ratio: np.ndarray | None = None) -> np.n
     if pw_ratio is not None:
         f *= np.asarray(pw_ratio)[: lmax + 1]
     return f
+
+
+# --------------------------------------------------------------------------
+# Counter-based generator of the device RNG (no counterpart in the reference,
+# which draws from numpy's PCG64; see rng.py of the package).
+
+_PHILOX_M0, _PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
+_PHILOX_W0, _PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
+
+
+def philox4x32_10(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
+    """Philox4x32 with 10 rounds (Salmon, Moraes, Dror, Shaw, SC'11):
+    ctr (..., 4) and key (..., 2) uint32 -> (..., 4) uint32."""
+    c = [np.asarray(ctr[..., i], dtype=np.uint64) for i in range(4)]
+    k0 = np.asarray(key[..., 0], dtype=np.uint64)
+    k1 = np.asarray(key[..., 1], dtype=np.uint64)
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(_PHILOX_M0) * c[0]
+        p1 = np.uint64(_PHILOX_M1) * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c = [(hi1 ^ c[1] ^ k0) & mask, lo1, (hi0 ^ c[3] ^ k1) & mask, lo0]
+        k0 = (k0 + np.uint64(_PHILOX_W0)) & mask
+        k1 = (k1 + np.uint64(_PHILOX_W1)) & mask
+    return np.stack([x.astype(np.uint32) for x in c], axis=-1)
+
+
+def device_normals(seed: int, shell: int, g: np.ndarray) -> np.ndarray:
+    """Complex standard normals of the device generator for GLASS indices g of
+    one shell: counter (g lo, g hi, shell lo, shell hi), key (seed lo, seed hi),
+    Box-Muller on u1 = (w0 + 1/2) 2^-32 and u2 = (w1 >> 8) 2^-24 (FP64 here; the
+    device evaluates log / sqrt / sin / cos in FP32)."""
+    g = np.asarray(g, dtype=np.uint64)
+    ctr = np.stack([g & np.uint64(0xFFFFFFFF), g >> np.uint64(32),
+                    np.full_like(g, shell & 0xFFFFFFFF), np.full_like(g, shell >> 32)], axis=-1)
+    key = np.broadcast_to(np.array([seed & 0xFFFFFFFF, seed >> 32], dtype=np.uint64), ctr.shape[:-1] + (2,))
+    w = philox4x32_10(ctr.astype(np.uint32), key.astype(np.uint32))
+    u1 = (np.float32(w[..., 0]).astype(np.float64) + 0.5) * 2.0**-32
+    u2 = (w[..., 1] >> 8).astype(np.float64) * 2.0**-24
+    rad = np.sqrt(-2.0 * np.log(u1))
+    return rad * np.cos(2 * np.pi * u2) + 1j * rad * np.sin(2 * np.pi * u2)

@@ -226,3 +226,22 @@ def test_from_convergence_chain():
     assert np.max(np.abs(g1 + 1j * g2 - gamma)) / np.sqrt(np.mean(np.abs(gamma) ** 2)) < 1e-10
     with pytest.raises(NotImplementedError, match="pixwin_ratio"):
         glass_amd.from_convergence(kappa, lmax, shear=True)
+
+
+@pytest.mark.parametrize("shell,seed", [(0, 7), (5, 2**40 + 3)])
+def test_device_normals_match_oracle(shell, seed):
+    """The device generator's normals (k = 0, w = 1: alm = healpy-ordered z
+    with the m = 0 fix) against the oracle's Philox4x32-10 + Box-Muller; the
+    device evaluates the transcendental functions in FP32."""
+    from glass_amd import _lib
+
+    nl = 40
+    z = O.device_normals(seed, shell, np.arange(nl * (nl + 1) // 2))
+    ref = O.glass_to_healpix_alm(z)
+    ref[:nl] = ref[:nl].real + ref[:nl].imag
+    w = torch.ones((1, nl, 1), dtype=torch.float64, device="cuda")
+    out = torch.empty(nl * (nl + 1) // 2, dtype=torch.complex128, device="cuda")
+    _lib.call("ga_fields_alm", nl, 0, shell, 1, _lib.ptr(w), None, 1, seed, None, 0,
+              _lib.ptr(out), _lib.stream_ptr())
+    got = out.cpu().numpy()
+    assert np.max(np.abs(got - ref)) < 2e-5

@@ -388,3 +388,22 @@ def test_generate_invariants():
     np.testing.assert_allclose(res[1], res[0] ** 2, atol=1e-5)
     with pytest.raises(ValueError, match="mismatch between number of fields and gls"):
         list(O.generate(fields, [np.ones(10), np.ones(10)], nside=4))
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 against the generator's published known-answer vectors
+    (Random123 distribution, kat_vectors: counter, key -> output)."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF),
+            (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    ctr = np.array([c for c, _, _ in kat], dtype=np.uint32)
+    key = np.array([k for _, k, _ in kat], dtype=np.uint32)
+    out = O.philox4x32_10(ctr, key)
+    assert out.tolist() == [list(e) for _, _, e in kat]
+    # Box-Muller on top: standard complex normals
+    z = O.device_normals(11, 2, np.arange(200000))
+    assert abs(z.real.mean()) < 0.02 and abs(z.imag.mean()) < 0.02
+    assert abs(z.real.var() - 1) < 0.02 and abs(z.imag.var() - 1) < 0.02
+    assert abs(np.mean(z.real * z.imag)) < 0.02
