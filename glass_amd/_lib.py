@@ -32,6 +32,8 @@ TUNE_MFMA_TILES = 1
 TUNE_MFMA_CHUNK = 2
 TUNE_MFMA_PATH = 3
 TUNE_RING_PATH = 4
+TUNE_SPIN_PATH = 5
+TUNE_ANAL_PATH = 6
 
 T_NORMAL = 0
 T_LOGNORMAL = 1

@@ -80,7 +80,6 @@ class ShearShells:
                      self.plan.workspace_bytes(2 * self.S, 4))
         self.ws = _lib.workspace(nbytes)
         self.alm = T.empty((self.nalm, self.S), dtype=T.complex64, device="cuda")
-        self.zero = T.zeros_like(self.alm)
         self.tmp = T.empty((self.S, self.npix), dtype=T.float32, device="cuda")
         self.fl = T.as_tensor(shear_factor(self.lmax, pixwin_ratio)).to("cuda")
         self.gamma = T.empty((self.S, 2, self.npix), dtype=T.float32, device="cuda")
@@ -95,8 +94,9 @@ class ShearShells:
                   int(self.fl.numel()), _lib.ptr(self.alm), _lib.stream_ptr())
 
     def stage_spin(self):
+        # almB = NULL: B = 0, one contraction
         _lib.call("ga_alm2map_spin_batch_f32", self.plan.h, 2, self.S, _lib.ptr(self.alm),
-                  _lib.ptr(self.zero), _lib.ptr(self.gamma), _lib.ptr(self.ws), self.ws.numel(),
+                  None, _lib.ptr(self.gamma), _lib.ptr(self.ws), self.ws.numel(),
                   _lib.stream_ptr())
 
     def run(self, kappa):

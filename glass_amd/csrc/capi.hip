@@ -188,11 +188,18 @@ extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t
                                          const float *E, const float *B, float *maps, void *ws,
                                          size_t ws_bytes, void *stream) {
   GA_PLAN_OK(p);
-  GA_REQUIRE(S >= 1 && E && B && maps && ws, GA_ERR_INVALID, "invalid argument");
+  GA_REQUIRE(S >= 1 && E && maps && ws, GA_ERR_INVALID, "invalid argument");
   GA_REQUIRE(ws_bytes >= ring_bytes(p, 2 * S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
              ws_bytes, ring_bytes(p, 2 * S, 4));
+  GA_REQUIRE(spin == 1 || spin == 2, GA_ERR_UNSUPPORTED, "spin %d not supported (1 or 2)", spin);
   hipStream_t s = (hipStream_t)stream;
-  int rc = launch_spin_synth_f32(p, spin, S, E, B, (float *)ws, s);
+  if (tuning().spin_path == 1) {
+    GA_REQUIRE(B, GA_ERR_INVALID, "the VALU spin path needs almB");
+    int rc = launch_spin_synth_f32(p, spin, S, E, B, (float *)ws, s);
+    if (rc) return rc;
+    return launch_ring_synth_f32(p, 2 * S, 2 * S, (const float *)ws, maps, nullptr, s);
+  }
+  int rc = launch_spin_synth_f16(p, spin, S, E, B, ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, 2 * S, 2 * S, (const float *)ws, maps, nullptr, s);
+  return launch_ring_synth_f32(p, 2 * S, kMfmaGroup, (const float *)ws, maps, nullptr, s);
 }

@@ -79,6 +79,46 @@ __device__ __forceinline__ void legendre_seed(double cm, double sth, int m, doub
   p = ldexp(cm * rm, rem);
 }
 
+// Seeds of the spin-s recurrences at l0 = max(m, s) for one ring:
+// +s: K sin^{m+s}(t/2) cos^{|m-s|}(t/2), -s: K' sin^{|m-s|}(t/2) cos^{m+s}(t/2),
+// with K = mant 2^exp (signed mantissa) from the plan's seed table.  The powers
+// are formed as mantissa and exact binary exponent: b^n = mant 2^e.
+__device__ __forceinline__ void scaled_power(double b, int n, double &mant, long long &e) {
+  int eb;
+  const double bm0 = frexp(b, &eb);
+  double rm = 1.0, bm = bm0;
+  long long er = 0, ebb = 0;
+  int nn = n;
+  while (nn) {
+    if (nn & 1) {
+      int t;
+      rm = frexp(rm * bm, &t);
+      er += ebb + t;
+    }
+    nn >>= 1;
+    if (nn) {
+      int t;
+      bm = frexp(bm * bm, &t);
+      ebb = 2 * ebb + t;
+    }
+  }
+  mant = rm;
+  e = er + (long long)n * eb;
+}
+
+// value = mant 2^E (E <= 0 in use) in the scaled form p 2^(600 k), k <= 0
+__device__ __forceinline__ void to_scaled(double mant, long long E, double &p, int &k) {
+  if (E > 0) {  // cannot exceed O(1) values, but keep the representation valid
+    p = ldexp(mant, int(E));
+    k = 0;
+    return;
+  }
+  const long long negE = -E;
+  k = -int(negE / kScaleBits);
+  const int rem = int(E + (long long)kScaleBits * (negE / kScaleBits));
+  p = ldexp(mant, rem);
+}
+
 // full 64-bit lane masks
 __device__ __forceinline__ bool wave_any(bool v) { return __any(v); }
 __device__ __forceinline__ bool wave_all(bool v) { return __all(v); }

@@ -1,0 +1,106 @@
+// Split-FP16 MFMA pieces shared by the scalar and spin Legendre stages (gfx950).
+//
+// An FP32 operand x is carried as x = x_hi + x_lo, two FP16 numbers (22
+// significant bits), and a product as three v_mfma_f32_32x32x16_f16:
+//     lambda a ~ lambda_hi a_hi + lambda_hi a_lo + lambda_lo a_hi.
+// Operand layouts of v_mfma_f32_32x32x16_f16 (wave64): lane L holds
+// A[L % 32][8 (L / 32) + j] and B[8 (L / 32) + j][L % 32], j = 0..7; the
+// accumulator element e of lane L is D[8 (e / 4) + 4 (L / 32) + e % 4][L % 32].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ga {
+namespace f16 {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kLamExp = 8;  // recurrence values are carried times 2^8
+
+__device__ __forceinline__ uint32_t bits_h2(f16x2 h) { return __builtin_bit_cast(uint32_t, h); }
+
+// a - (the low / high FP16 half of h, widened): one v_fma_mix_f32 (-h * 1 + a)
+// with the half selected by op_sel; exact, since a - h is representable
+__device__ __forceinline__ float sub_half_lo(float a, uint32_t h) {
+  float r;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
+  return r;
+}
+__device__ __forceinline__ float sub_half_hi(float a, uint32_t h) {
+  float r;
+  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h),
+      "v"(a));
+  return r;
+}
+
+// x = hi + lo for two values at once: hi rounded toward zero (one packed
+// conversion; its error is captured exactly by the residual), lo = the FP32
+// residual rounded to nearest.
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t &hi, uint32_t &lo) {
+  const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
+  hi = __builtin_bit_cast(uint32_t, h);
+  const float ra = sub_half_lo(a, hi), rb = sub_half_hi(b, hi);
+  f16x2 l;
+  l[0] = (_Float16)ra;
+  l[1] = (_Float16)rb;
+  lo = bits_h2(l);
+}
+
+// The MFMAs of one 32-row slab.  ph / pl [parity][group] hold a lane's 32
+// values of the slab as packed FP16 pairs (group g: k = 2g, 2g + 1 of the 16
+// rows of that parity); v_permlane32_swap(X, Y) = ([X_lo | Y_lo], [X_hi | Y_hi])
+// turns them into the A operands of two row tiles: tile 0 = the rows computed by
+// lanes 0..31, tile 1 = those of lanes 32..63.  bh / bl [parity][column tile]
+// are the pre-swizzled B operands.  acc [parity][row tile][column tile].  The
+// three products are interleaved so that dependent MFMAs are 4 NT - 1
+// instructions apart.
+template <int NT>
+__device__ __forceinline__ void slab_mfma(const uint32_t (&ph)[2][8], const uint32_t (&pl)[2][8],
+                                          const uint4 (&bh)[2][NT], const uint4 (&bl)[2][NT],
+                                          f32x16 (&acc)[2][2][NT]) {
+  uint32_t ah[2][2][4], al[2][2][4];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const auto sh = __builtin_amdgcn_permlane32_swap(ph[p][w], ph[p][w + 4], false, false);
+      const auto sl = __builtin_amdgcn_permlane32_swap(pl[p][w], pl[p][w + 4], false, false);
+      ah[p][0][w] = sh[0];
+      ah[p][1][w] = sh[1];
+      al[p][0][w] = sl[0];
+      al[p][1][w] = sl[1];
+    }
+  f16x8 Ah[2][2], Al[2][2], Bh[2][NT], Bl[2][NT];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      Ah[p][t] = __builtin_bit_cast(f16x8, uint4{ah[p][t][0], ah[p][t][1], ah[p][t][2], ah[p][t][3]});
+      Al[p][t] = __builtin_bit_cast(f16x8, uint4{al[p][t][0], al[p][t][1], al[p][t][2], al[p][t][3]});
+    }
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) {
+      Bh[p][ct] = __builtin_bit_cast(f16x8, bh[p][ct]);
+      Bl[p][ct] = __builtin_bit_cast(f16x8, bl[p][ct]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          const f16x8 a = (q == 2) ? Al[p][t] : Ah[p][t];
+          const f16x8 bb = (q == 1) ? Bl[p][ct] : Bh[p][ct];
+          acc[p][t][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb, acc[p][t][ct], 0, 0, 0);
+        }
+}
+
+}  // namespace f16
+}  // namespace ga

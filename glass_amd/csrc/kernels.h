@@ -14,8 +14,9 @@ int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float 
 // operand preparation (alm -> split, swizzled B operands and column
 // exponents: the bws / colexp parts of the FP32 workspace, f32_workspace) and
 // the contraction proper
+// (rows l < lmin of the B operands are zero)
 int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
-                                int32_t *colexp, hipStream_t s);
+                                int32_t *colexp, hipStream_t s, int lmin = 0);
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s);
 // the batched Legendre stage on the path chosen by the tuning knob; ws is the
@@ -47,6 +48,10 @@ int launch_legendre_anal_f64(const ga_plan *p, int S, const double *F, double *a
                              double *partial, bool accumulate, hipStream_t s);
 int launch_legendre_anal_f32(const ga_plan *p, int S, const float *F, float *alm, float *partial,
                              bool accumulate, hipStream_t s);
+// the split-FP16 MFMA form of the FP32 analysis contraction: F -> partial
+// sums [slot][idx][S] complex, one slot per 512 ring pairs (the caller adds them)
+int launch_legendre_anal_f16(const ga_plan *p, int S, const float *F, float *partial,
+                             hipStream_t s);
 
 // Spin synthesis Legendre stage: (E, B) [idx][S] -> ring buffer [ring][m][2S],
 // column 2s = Q_s, 2s + 1 = U_s
@@ -54,6 +59,12 @@ int launch_spin_synth_f64(const ga_plan *p, int spin, int S, const double *E, co
                           double *Fo, hipStream_t s);
 int launch_spin_synth_f32(const ga_plan *p, int spin, int S, const float *E, const float *B,
                           float *Fo, hipStream_t s);
+// Spin synthesis Legendre stage on split-FP16 MFMA: E (and B, may be NULL for
+// B = 0) [idx][S] complex64 -> the grouped ring buffer of 2S transforms (Q_s =
+// transform 2s, U_s = 2s + 1; groups of kMfmaGroup) at the start of the FP32
+// workspace ws of 2S transforms (f32_workspace(p, 2S))
+int launch_spin_synth_f16(const ga_plan *p, int spin, int S, const float *E, const float *B,
+                          void *ws, hipStream_t s);
 
 int plan_spin_tables(ga_plan *p, int s);
 

@@ -20,11 +20,26 @@ namespace ga {
 
 namespace {
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
+// Sum over the 64 lanes of a wave, complete in lane 63, with DPP lane moves
+// (no LDS crossbar): quads, half rows, rows, then the row broadcasts 15 / 31.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_add(float v) {
+  const int m = __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false);
+  return v + __int_as_float(m);
+}
+__device__ __forceinline__ float wave_sum_last(float v) {
+  v = dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_add<0x140, 0xF>(v);  // row_mirror: every lane holds its row's sum
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;                    // lane 63: the wave's sum
+}
+__device__ __forceinline__ double wave_sum_last(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  return v;  // every lane (the FP64 path keeps the exact butterfly order)
 }
 
 constexpr int kAnalRPL = 8;  // ring pairs per lane
@@ -91,11 +106,11 @@ __global__ __launch_bounds__(256) void legendre_anal_kernel(
     if (wave_any(any_on)) {
 #pragma unroll
       for (int cc = 0; cc < SC; ++cc) {
-        acc_re[cc] = wave_sum(acc_re[cc]);
-        acc_im[cc] = wave_sum(acc_im[cc]);
+        acc_re[cc] = wave_sum_last(acc_re[cc]);
+        acc_im[cc] = wave_sum_last(acc_im[cc]);
       }
     }
-    if (lane == 0) {
+    if (lane == 63) {
 #pragma unroll
       for (int cc = 0; cc < SC; ++cc)
         if (s0 + cc < S) out[int64_t(l - m) * S + cc] = cplx<T>{acc_re[cc], m == 0 ? T(0) : acc_im[cc]};
@@ -119,44 +134,6 @@ __global__ void reduce_partials_kernel(int64_t n, int nw, const cplx<T> *__restr
 // spin-s synthesis: Q_m = sum_l E F+ + i B F-,  U_m = sum_l B F+ - i E F-,
 // F+- = -(sY +- (-1)^s (-s)Y) / 2; south ring via F+(pi - t) = sigma F+(t),
 // F-(pi - t) = -sigma F-(t), sigma = (-1)^(l+m+s).
-
-// Seeds of the spin recurrences at l0 = max(m, s) for one ring, as scaled values:
-// +s: K sin^{m+s}(t/2) cos^{|m-s|}(t/2), -s: K' sin^{|m-s|}(t/2) cos^{m+s}(t/2)
-// where (kmant, kexp) hold K = kmant 2^kexp with sign.
-__device__ __forceinline__ void scaled_power(double b, int n, double &mant, long long &e) {
-  int eb;
-  const double bm0 = frexp(b, &eb);
-  double rm = 1.0, bm = bm0;
-  long long er = 0, ebb = 0;
-  int nn = n;
-  while (nn) {
-    if (nn & 1) {
-      int t;
-      rm = frexp(rm * bm, &t);
-      er += ebb + t;
-    }
-    nn >>= 1;
-    if (nn) {
-      int t;
-      bm = frexp(bm * bm, &t);
-      ebb = 2 * ebb + t;
-    }
-  }
-  mant = rm;
-  e = er + (long long)n * eb;
-}
-
-__device__ __forceinline__ void to_scaled(double mant, long long E, double &p, int &k) {
-  if (E > 0) {  // cannot exceed O(1) values, but keep the representation valid
-    p = ldexp(mant, int(E));
-    k = 0;
-    return;
-  }
-  const long long negE = -E;
-  k = -int(negE / kScaleBits);
-  const int rem = int(E + (long long)kScaleBits * (negE / kScaleBits));
-  p = ldexp(mant, rem);
-}
 
 template <typename T, int SC>
 __global__ __launch_bounds__(256) void spin_synth_kernel(
@@ -205,9 +182,9 @@ __global__ __launch_bounds__(256) void spin_synth_kernel(
     const int64_t base = hp_index(lmax, m, m);
     for (int l = l0; l <= lmax; ++l) {
       const double *cf = tab + 3 * (base + (l - m));
-      const double A = cf[0], Bc = cf[1], C = cf[2];
-      double n1 = A * (x + Bc) * a1 - C * a0;
-      double n2 = A * (x - Bc) * b1 - C * b0;
+      const double A = cf[0], AB = cf[1], C = cf[2];
+      const double n1 = fma(fma(A, x, AB), a1, -(C * a0));
+      const double n2 = fma(fma(A, x, -AB), b1, -(C * b0));
       a0 = a1; a1 = n1;
       b0 = b1; b1 = n2;
       if (kp_ < 0 && fabs(a1) > 0x1p300) { a1 *= 0x1p-600; a0 *= 0x1p-600; ++kp_; }
@@ -260,13 +237,18 @@ template <typename T>
 static int launch_anal(const ga_plan *p, int S, const T *F, T *alm, T *partial, bool accumulate,
                        hipStream_t stream) {
   const int nw = anal_partial_slots(p);
-  const int nblk_y = (nw + 3) / 4;
-  constexpr int SC = sizeof(T) == 8 ? 1 : 2;
-  dim3 grid(unsigned(p->lmax + 1), unsigned(nblk_y), unsigned((S + SC - 1) / SC));
-  hipLaunchKernelGGL((legendre_anal_kernel<T, SC>), grid, dim3(256), 0, stream, p->lmax,
-                     p->npairs, p->nrings, S, p->d_z, p->d_sth, p->d_ab, p->d_cm,
-                     (const cplx<T> *)F, (cplx<T> *)partial, p->nalm);
-  GA_LAUNCH_CHECK();
+  if (sizeof(T) == 4 && tuning().anal_path != 1) {
+    int rc = launch_legendre_anal_f16(p, S, (const float *)F, (float *)partial, stream);
+    if (rc) return rc;
+  } else {
+    const int nblk_y = (nw + 3) / 4;
+    constexpr int SC = sizeof(T) == 8 ? 1 : 2;
+    dim3 grid(unsigned(p->lmax + 1), unsigned(nblk_y), unsigned((S + SC - 1) / SC));
+    hipLaunchKernelGGL((legendre_anal_kernel<T, SC>), grid, dim3(256), 0, stream, p->lmax,
+                       p->npairs, p->nrings, S, p->d_z, p->d_sth, p->d_ab, p->d_cm,
+                       (const cplx<T> *)F, (cplx<T> *)partial, p->nalm);
+    GA_LAUNCH_CHECK();
+  }
   const int64_t n = p->nalm * S;
   hipLaunchKernelGGL(reduce_partials_kernel<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
                      stream, n, nw, (const cplx<T> *)partial, (cplx<T> *)alm, accumulate ? 1 : 0);

@@ -1,0 +1,261 @@
+// Batched Legendre stage of the analysis (map2alm) on split-FP16 MFMA (gfx950).
+//
+//   a_lm[c] = sum_rp lambda_lm(theta_rp) Q_p[rp, c],   Q_0 = F_N + F_S, Q_1 = F_N - F_S,
+//
+// p = parity of l - m, c = real column (re / im of a shell), F the weighted,
+// phase-removed ring coefficients [ring][m][c].  The sum runs over ring pairs,
+// which is the K dimension of the MFMA: D[l][c] += A[l][rp] B[rp][c] with
+// A = lambda (split FP16, times 2^8) and B = Q_p (split FP16, scaled per column
+// and wave into [2^12, 2^13)), three v_mfma_f32_32x32x16_f16 per product as in
+// the synthesis stages.
+//
+// Work decomposition: a workgroup = 8 waves = 512 ring pairs of one m and one
+// 32-column tile; a lane owns one ring pair and runs its FP64 recurrence.  The
+// l-range of m is cut into slabs of 64 rows (32 per parity, one 32-row MFMA
+// tile each).  The recurrence yields lambda with ring pairs in lanes and l in
+// registers, but the MFMA wants l in lanes (rows) and ring pairs in registers
+// (K): each wave transposes its slab through its own LDS tile
+// [hi|lo][parity][l'][pair] (16-bit stores of a lane's values, 16-byte loads
+// of 8 consecutive pairs).  B (the wave's 64 pairs x 32 columns, both
+// parities) stays in registers for the whole m.  After each slab the 8 waves'
+// D tiles are summed in LDS in a fixed order and the workgroup writes its
+// partial sum [slot][idx][c]; a second kernel adds the slots in a fixed order,
+// so the result does not depend on scheduling.
+#include "common.h"
+#include "device.h"
+#include "f16.h"
+#include "kernels.h"
+
+namespace ga {
+
+namespace {
+
+using f16::f32x16;
+using f16::f16x8;
+using f16::kLamExp;
+using f16::split_pair;
+
+constexpr int kAWaves = 8;                // waves per workgroup
+constexpr int kAPairs = 64 * kAWaves;     // ring pairs per workgroup = one partial slot
+constexpr int kASlab = 64;                // l rows per slab
+constexpr int kAColTop = 13;              // column maxima are scaled below 2^13
+
+__device__ __forceinline__ int anal_shift(float mx) {
+  if (!(mx > 0.0f) || !isfinite(mx)) return 0;
+  int e;
+  (void)frexpf(mx, &e);  // mx < 2^e
+  return kAColTop - e;
+}
+
+__global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
+    int lmax, int npairs, int nrings, int ncols, int nrg, int ncg, const double *__restrict__ z,
+    const double *__restrict__ sth, const double2 *__restrict__ aab,
+    const double *__restrict__ cm, const float *__restrict__ F, float *__restrict__ partial,
+    int64_t nalm) {
+  // XCD-aware decode (as in the synthesis stages): XCD x takes m = x, x + 8, ...
+  const int b = blockIdx.x;
+  const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
+  const int m = 8 * (kq / G) + xcd;
+  if (m > lmax) return;  // whole workgroup
+  const int rem = kq - (kq / G) * G;
+  const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int half = lane >> 5, c32 = lane & 31;
+  const int pw0 = rg * kAPairs + wave * 64;  // first ring pair of this wave
+  const int rp = pw0 + lane;
+  const bool valid = rp < npairs;
+  const int rpc = valid ? rp : npairs - 1;
+  const double x = z[rpc];
+  double p0, p1 = 0.0;
+  int k;
+  legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
+  if (!valid) {  // a missing ring pair contributes zeros
+    p0 = 0.0;
+    k = 0;
+  }
+
+  // B operands: lane L holds B[k = 8 (L/32) + j][n = L % 32] of k-block kb,
+  // i.e. ring pair pw0 + 16 kb + 8 half + j, column col, for both parities.
+  const int col = cg * 32 + c32;
+  const bool colok = col < ncols;
+  uint4 bh[2][4], bl[2][4];
+  float fac;
+  {
+    float fn[4][8], fs[4][8];
+    float mx = 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int pr = pw0 + 16 * kb + 8 * half + j;
+        float vn = 0.0f, vs = 0.0f;
+        if (colok && pr < npairs) {
+          vn = F[(int64_t(pr) * (lmax + 1) + m) * ncols + col];
+          if (pr < npairs - 1) vs = F[(int64_t(nrings - 1 - pr) * (lmax + 1) + m) * ncols + col];
+        }
+        fn[kb][j] = vn;
+        fs[kb][j] = vs;
+        mx = fmaxf(mx, fmaxf(fabsf(vn + vs), fabsf(vn - vs)));
+      }
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));  // lanes L and L ^ 32 hold the same column
+    const int sh = anal_shift(mx);
+    fac = colok ? ldexpf(1.0f, -(kLamExp + sh)) : 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      uint32_t h[2][4], l[2][4];
+#pragma unroll
+      for (int j = 0; j < 8; j += 2) {
+        const float e0 = ldexpf(fn[kb][j] + fs[kb][j], sh), e1 = ldexpf(fn[kb][j + 1] + fs[kb][j + 1], sh);
+        const float o0 = ldexpf(fn[kb][j] - fs[kb][j], sh), o1 = ldexpf(fn[kb][j + 1] - fs[kb][j + 1], sh);
+        split_pair(e0, e1, h[0][j / 2], l[0][j / 2]);
+        split_pair(o0, o1, h[1][j / 2], l[1][j / 2]);
+      }
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        bh[p][kb] = uint4{h[p][0], h[p][1], h[p][2], h[p][3]};
+        bl[p][kb] = uint4{l[p][0], l[p][1], l[p][2], l[p][3]};
+      }
+    }
+  }
+
+  // LDS: per wave the transposed slab lam[hi|lo][parity][l' (32)][pair (64)]
+  // (16 KB); after the MFMAs of a slab the same memory holds the waves' D
+  // tiles for the fixed-order sum.  Recurrence coefficients (a, a b) go
+  // through a per-wave double buffer of one slab.
+  __shared__ __align__(16) _Float16 lam[kAWaves][2][2][32][64];
+  __shared__ double2 cbuf[kAWaves][2][kASlab];
+  float *red = reinterpret_cast<float *>(&lam[0][0][0][0][0]);  // [wave][p][e][lane]
+
+  const int64_t base = hp_index(lmax, m, m);
+  const double2 *__restrict__ co = aab + base;
+  const int lastc = lmax - m;
+  double2 cpre = co[min(lane, lastc)];
+  cbuf[wave][0][lane] = cpre;
+  __builtin_amdgcn_wave_barrier();
+  _Float16(*lw)[2][32][64] = lam[wave];  // [hi|lo][p][l'][pair]
+
+  const int nsl = (lmax - m + kASlab) / kASlab;
+  for (int s = 0; s < nsl; ++s) {
+    const int l0 = m + kASlab * s;
+    const int cur = s & 1;
+    cpre = co[min(kASlab * (s + 1) + lane, lastc)];
+    const double2 *__restrict__ cb = cbuf[wave][cur];
+    const bool slow = (l0 + kASlab - 1 > lmax) || wave_any(k < 0);
+    // 16 groups of 4 steps: step i -> parity i & 1, row l' = i >> 1
+#pragma unroll 2
+    for (int g = 0; g < 16; ++g) {
+      float v[4];
+      if (!slow) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double2 c = cb[4 * g + j];
+          const double pn = fma(c.x * x, p1, -(c.y * p0));
+          p0 = p1;
+          p1 = pn;
+          v[j] = float(pn);
+        }
+      } else {
+        const int lg = l0 + 4 * g;
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        if (lg <= lmax) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const double2 c = cb[4 * g + j];
+            const double pn = fma(c.x * x, p1, -(c.y * p0));
+            p0 = p1;
+            p1 = pn;
+            d[j] = pn;
+          }
+          if (k != 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) d[j] = 0.0;
+          }
+          if (k < 0 && fabs(p1) > 0x1p300) {
+            p1 *= 0x1p-600;
+            p0 *= 0x1p-600;
+            ++k;
+          }
+          // rows past lmax are computed but never stored: no masking needed
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = float(d[j]);
+      }
+      uint32_t h0, l0w, h1, l1w;
+      split_pair(v[0], v[2], h0, l0w);  // parity 0, rows 2g, 2g + 1
+      split_pair(v[1], v[3], h1, l1w);  // parity 1
+      const f16::f16x2 H0 = __builtin_bit_cast(f16::f16x2, h0), L0 = __builtin_bit_cast(f16::f16x2, l0w);
+      const f16::f16x2 H1 = __builtin_bit_cast(f16::f16x2, h1), L1 = __builtin_bit_cast(f16::f16x2, l1w);
+      lw[0][0][2 * g][lane] = H0[0];
+      lw[0][0][2 * g + 1][lane] = H0[1];
+      lw[1][0][2 * g][lane] = L0[0];
+      lw[1][0][2 * g + 1][lane] = L0[1];
+      lw[0][1][2 * g][lane] = H1[0];
+      lw[0][1][2 * g + 1][lane] = H1[1];
+      lw[1][1][2 * g][lane] = L1[0];
+      lw[1][1][2 * g + 1][lane] = L1[1];
+    }
+    cbuf[wave][cur ^ 1][lane] = cpre;
+    __builtin_amdgcn_wave_barrier();
+
+    // MFMAs: A[l'][pair] of k-block kb: lane L reads row L % 32, pairs
+    // 16 kb + 8 (L / 32) .. + 7 (16 bytes)
+    f32x16 acc[2] = {f32x16{}, f32x16{}};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const f16x8 ah = *reinterpret_cast<const f16x8 *>(&lw[0][p][c32][16 * kb + 8 * half]);
+        const f16x8 al = *reinterpret_cast<const f16x8 *>(&lw[1][p][c32][16 * kb + 8 * half]);
+        const f16x8 Bh = __builtin_bit_cast(f16x8, bh[p][kb]);
+        const f16x8 Bl = __builtin_bit_cast(f16x8, bl[p][kb]);
+        acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, Bh, acc[p], 0, 0, 0);
+        acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, Bl, acc[p], 0, 0, 0);
+        acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, Bh, acc[p], 0, 0, 0);
+      }
+    }
+
+    // fixed-order sum of the 8 waves' tiles: element e of lane L is row
+    // 8 (e/4) + 4 (L/32) + e%4 (l = l0 + 2 row + p), column L % 32
+    __syncthreads();  // every wave has read its lambda tile
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) red[((wave * 2 + p) * 16 + e) * 64 + lane] = acc[p][e] * fac;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int pe = wave * 4 + q;  // (parity, element) handled by this wave
+      const int p = pe >> 4, e = pe & 15;
+      float sum = 0.0f;
+#pragma unroll
+      for (int w = 0; w < kAWaves; ++w) sum += red[((w * 2 + p) * 16 + e) * 64 + lane];
+      const int row = 8 * (e >> 2) + 4 * half + (e & 3);
+      const int l = l0 + 2 * row + p;
+      if (l <= lmax && colok)
+        partial[(int64_t(rg) * nalm + base + (l - m)) * ncols + col] =
+            (m == 0 && (col & 1)) ? 0.0f : sum;
+    }
+    __syncthreads();  // the sums are read before the next slab's lambda tile
+  }
+}
+
+}  // namespace
+
+int launch_legendre_anal_f16(const ga_plan *p, int S, const float *F, float *partial,
+                             hipStream_t stream) {
+  const int ncols = 2 * S;
+  const int nrg = (p->npairs + kAPairs - 1) / kAPairs;
+  GA_REQUIRE(nrg == anal_partial_slots(p), GA_ERR_UNSUPPORTED, "partial slots %d != %d", nrg,
+             anal_partial_slots(p));
+  const int ncg = (ncols + 31) / 32;
+  const int nm8 = (p->lmax + 1 + 7) / 8;
+  const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
+  GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
+  hipLaunchKernelGGL(legendre_anal_f16_kernel, dim3(unsigned(nblk)), dim3(64 * kAWaves), 0,
+                     stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth,
+                     p->d_aab, p->d_cm, F, partial, p->nalm);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+}  // namespace ga

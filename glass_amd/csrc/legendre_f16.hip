@@ -28,47 +28,18 @@
 // for the low part), prepared once per batch by the two small kernels below.
 #include "common.h"
 #include "device.h"
+#include "f16.h"
 #include "kernels.h"
 
 namespace ga {
 
 namespace {
 
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+using f16::f32x16;
+using f16::kLamExp;
+using f16::split_pair;
 
-constexpr int kLamExp = 8;   // lambda is carried as lambda 2^8
 constexpr int kColTop = 13;  // column maxima are scaled below 2^13
-
-__device__ __forceinline__ uint32_t bits_h2(f16x2 h) { return __builtin_bit_cast(uint32_t, h); }
-
-// a - (the low / high FP16 half of h, widened): one v_fma_mix_f32 (-h * 1 + a)
-// with the half selected by op_sel; exact, since a - h is representable
-__device__ __forceinline__ float sub_half_lo(float a, uint32_t h) {
-  float r;
-  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
-  return r;
-}
-__device__ __forceinline__ float sub_half_hi(float a, uint32_t h) {
-  float r;
-  asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h),
-      "v"(a));
-  return r;
-}
-
-// x = hi + lo for two values at once: hi rounded toward zero (one packed
-// conversion; its error is captured exactly by the residual), lo = the FP32
-// residual rounded to nearest.
-__device__ __forceinline__ void split_pair(float a, float b, uint32_t &hi, uint32_t &lo) {
-  const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
-  hi = __builtin_bit_cast(uint32_t, h);
-  const float ra = sub_half_lo(a, hi), rb = sub_half_hi(b, hi);
-  f16x2 l;
-  l[0] = (_Float16)ra;
-  l[1] = (_Float16)rb;
-  lo = bits_h2(l);
-}
 
 __global__ void fill_u32_kernel(uint32_t *__restrict__ p, int n, uint32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -142,7 +113,8 @@ __device__ __forceinline__ int col_shift(uint32_t bits) {
 // unit holds rows l0 + p + 16 (L / 32) + 2 j (j = 0..7) of column tile * 32 +
 // L % 32, as 8 high halves (first 64 x 16 bytes) and 8 low halves (next).
 // grid: (ceil(units of m = 0 / 4), lmax + 1): blockIdx.y = m, 4 units per block.
-__global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int ncols, int nct,
+// Rows l < lmin are stored as zero (the spin-s stages: no modes below l = s).
+__global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int ncols, int nct,
                                                         const int64_t *__restrict__ slab_off,
                                                         const float *__restrict__ alm,
                                                         const uint32_t *__restrict__ cmax,
@@ -169,7 +141,8 @@ __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int ncols, int
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int l = l0 + p + 16 * (L >> 5) + 2 * (j + q);
-      v[q] = (c < ncols && l <= lmax) ? ldexpf(col[int64_t(l - m) * ncols], sh) : 0.0f;
+      v[q] = (c < ncols && l <= lmax && l >= lmin) ? ldexpf(col[int64_t(l - m) * ncols], sh)
+                                                    : 0.0f;
     }
     split_pair(v[0], v[1], hv[j / 2], lv[j / 2]);
   }
@@ -256,22 +229,31 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
     uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
     const bool slow = (l0 + kSlab - 1 > lmax) || wave_any(k < 0);
     if (!slow) {
-      // every lane in range and a full slab: the plain recurrence
+      // every lane in range and a full slab: the plain recurrence.  The
+      // coefficients of group g + 1 are read from LDS while group g runs; the
+      // scheduling barrier keeps later groups' reads from being hoisted
+      // further (each would hold 8 VGPRs for the rest of the slab).
+      double2 cc[4], cn[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cc[j] = cb[j];
 #pragma unroll
       for (int g = 0; g < 8; ++g) {
+        if (g < 7) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) cn[j] = cb[4 * (g + 1) + j];
+        }
         float v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const double2 c = cb[4 * g + j];
-          const double pn = fma(c.x * x, p1, -(c.y * p0));
+          const double pn = fma(cc[j].x * x, p1, -(cc[j].y * p0));
           p0 = p1;
           p1 = pn;
           v[j] = float(pn);
         }
         split_pair(v[0], v[2], ph[0][g], pl[0][g]);
         split_pair(v[1], v[3], ph[1][g], pl[1][g]);
-        // keep the coefficient loads of later groups from being hoisted here
-        // (they would hold 4 VGPRs each for the whole slab)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cc[j] = cn[j];
         __builtin_amdgcn_sched_barrier(0);
       }
     } else {
@@ -322,51 +304,7 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
 
     // MFMAs only if some lane has a nonzero lambda in this slab (k only
     // grows, so a lane still scaled at the end of the slab contributed nothing)
-    if (wave_any(k == 0)) {
-      // A operands: words 0..3 = groups 0..3 (k 0..7), words 4..7 = k 8..15;
-      // swap(X, Y) gives tile 0 = [own X | partner's Y], tile 1 = [partner's X | own Y]
-      uint32_t ah[2][2][4], al[2][2][4];
-#pragma unroll
-      for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          const auto sh = __builtin_amdgcn_permlane32_swap(ph[p][w], ph[p][w + 4], false, false);
-          const auto sl = __builtin_amdgcn_permlane32_swap(pl[p][w], pl[p][w + 4], false, false);
-          ah[p][0][w] = sh[0];
-          ah[p][1][w] = sh[1];
-          al[p][0][w] = sl[0];
-          al[p][1][w] = sl[1];
-        }
-      f16x8 Ah[2][2], Al[2][2], Bh[2][NT], Bl[2][NT];
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          Ah[p][t] = __builtin_bit_cast(f16x8, uint4{ah[p][t][0], ah[p][t][1], ah[p][t][2], ah[p][t][3]});
-          Al[p][t] = __builtin_bit_cast(f16x8, uint4{al[p][t][0], al[p][t][1], al[p][t][2], al[p][t][3]});
-        }
-#pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
-          Bh[p][ct] = __builtin_bit_cast(f16x8, bh[p][ct]);
-          Bl[p][ct] = __builtin_bit_cast(f16x8, bl[p][ct]);
-        }
-      }
-      // the three products, accumulators interleaved so that dependent MFMAs
-      // are 4 NT - 1 instructions apart; both row tiles whenever either is on
-      // (straight-line code; the matrix pipe has slack next to the recurrence)
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int ct = 0; ct < NT; ++ct) {
-              const f16x8 a = (q == 2) ? Al[p][t] : Ah[p][t];
-              const f16x8 bb = (q == 1) ? Bl[p][ct] : Bh[p][ct];
-              acc[p][t][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb, acc[p][t][ct], 0, 0, 0);
-            }
-    }
+    if (wave_any(k == 0)) f16::slab_mfma<NT>(ph, pl, bh, bl, acc);
   }
 
   // epilogue (as in the FP32 stage): D layout of 32x32: lane L, element e ->
@@ -424,7 +362,7 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
 }  // namespace
 
 int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
-                                int32_t *colexp, hipStream_t stream) {
+                                int32_t *colexp, hipStream_t stream, int lmin) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   uint4 *B = reinterpret_cast<uint4 *>(bws);
@@ -445,8 +383,8 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
   GA_REQUIRE(p->lmax < 65535, GA_ERR_UNSUPPORTED, "lmax too large for the operand preparation");
   const int units0 = int(nslab_of(p->lmax, 0)) * 2 * nct;
   hipLaunchKernelGGL(swizzle_b_kernel, dim3(unsigned((units0 + 3) / 4), unsigned(p->lmax + 1)),
-                     dim3(256), 0, stream, p->lmax, ncols, nct, p->d_slab_off, alm, cmax, B,
-                     colexp);
+                     dim3(256), 0, stream, p->lmax, lmin, ncols, nct, p->d_slab_off, alm, cmax,
+                     B, colexp);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

@@ -126,8 +126,11 @@ int upload(T **dst, const void *src, size_t bytes, size_t *acc) {
 }  // namespace
 
 // Spin recurrence tables, built on first use: per (l, m) in alm order,
-// (A, B, C) with  sl_{l} = A (x + B) sl_{l-1} - C sl_{l-2}  (+s) and
-// (x - B) for -s, from the Wigner-d three-term recurrence.
+// (A, A B, C) with  sl_{l} = (A x + A B) sl_{l-1} - C sl_{l-2}  (+s) and
+// (A x - A B) for -s, from the Wigner-d three-term recurrence.  The entries
+// for m <= l <= l0 = max(m, s) are (0, 0, -1): such a step moves sl_{l-2}
+// into place, so a recurrence that starts at l = m with the seed in the right
+// slot (which depends on the parity of l0 - m) reaches sl_{l0} = seed.
 int plan_spin_tables(ga_plan *p, int s) {
   if (p->d_spin[s]) return GA_OK;
   const int64_t lmax = p->lmax, nalm = p->nalm;
@@ -135,9 +138,9 @@ int plan_spin_tables(ga_plan *p, int s) {
   std::vector<double2> seeds(size_t(2 * (lmax + 1)), double2{0.0, 0.0});
   for (int64_t m = 0; m <= lmax; ++m) {
     int64_t l0 = m > s ? m : s;
-    // (0, 0, -1) at l = l0: one step from (seed, 0) yields the seed itself
-    if (l0 <= lmax) {
-      size_t i0 = size_t(3 * hp_index(lmax, l0, m));
+    // (0, 0, -1) at l = m .. l0: one step from (seed, 0) yields the seed itself
+    for (int64_t l = m; l <= l0 && l <= lmax; ++l) {
+      size_t i0 = size_t(3 * hp_index(lmax, l, m));
       t[i0] = 0.0; t[i0 + 1] = 0.0; t[i0 + 2] = -1.0;
     }
     // |seed constant| = sqrt((2 l0 + 1)/(4 pi) (2 l0)! / ((m + s)! |m - s|!)), as m 2^e
@@ -163,7 +166,7 @@ int plan_spin_tables(ga_plan *p, int s) {
       double C = std::sqrt(2 * L + 3) * (L + 1) * std::sqrt((L * L - mm * mm) * (L * L - ss * ss)) /
                  (L * std::sqrt(2 * L - 1) * den);
       size_t i = size_t(3 * hp_index(lmax, l, m));
-      t[i] = A; t[i + 1] = B; t[i + 2] = C;
+      t[i] = A; t[i + 1] = A * B; t[i + 2] = C;
     }
   }
   int rc = upload(&p->d_spin[s], t.data(), t.size() * sizeof(double), &p->table_bytes);
@@ -197,6 +200,14 @@ extern "C" int ga_set_tuning(int32_t key, int32_t value) {
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "ring path must be 0, 1 or 2");
       tuning().ring_path = value;
       return GA_OK;
+    case GA_TUNE_SPIN_PATH:
+      GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "spin path must be 0, 1 or 2");
+      tuning().spin_path = value;
+      return GA_OK;
+    case GA_TUNE_ANAL_PATH:
+      GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "analysis path must be 0, 1 or 2");
+      tuning().anal_path = value;
+      return GA_OK;
     default:
       GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
@@ -210,6 +221,8 @@ extern "C" int ga_get_tuning(int32_t key, int32_t *value) {
     case GA_TUNE_MFMA_CHUNK: *value = tuning().mfma_chunk; return GA_OK;
     case GA_TUNE_MFMA_PATH: *value = tuning().mfma_path; return GA_OK;
     case GA_TUNE_RING_PATH: *value = tuning().ring_path; return GA_OK;
+    case GA_TUNE_SPIN_PATH: *value = tuning().spin_path; return GA_OK;
+    case GA_TUNE_ANAL_PATH: *value = tuning().anal_path; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }

@@ -283,6 +283,130 @@ __device__ __forceinline__ bool decode_ring_block(int nrings, int npp, int &ring
 // (neighbouring lanes read neighbouring m: coalesced), does the radix-4 step
 // and twiddle in registers, then the four length-r DFTs run in LDS and the
 // thread stores pixels 4 j1 .. 4 j1 + 3.
+// Aliasing fold for the first ring kernel, per shell: bin k of shell q,
+//   G_k = sum_{m = k mod n} F_m + sum_{m = -k mod n, m > 0} conj(F_m)   (m <= lmax),
+// direct terms summed before conjugate terms, each in increasing m.
+// fold_shells_own: one thread, one bin; the loads of four terms are issued
+// together at clamped addresses (one round when n > lmax).
+template <typename T, int NQ>
+__device__ __forceinline__ void fold_shells_own(const cplx<T> *__restrict__ Fu, int G, int nsh,
+                                                int k, int n, int lmax, cplx<T> (&g)[NQ]) {
+  const int mc0 = k == 0 ? n : n - k;
+  if (n > lmax) {
+    cplx<T> fd[NQ], fc[NQ];
+    const cplx<T> *a = Fu + int64_t(min(k, lmax)) * G;
+    const cplx<T> *c = Fu + int64_t(min(mc0, lmax)) * G;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      fd[q] = a[min(q, nsh - 1)];
+      fc[q] = c[min(q, nsh - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (q >= nsh) continue;  // missing shells stay zero
+      if (k <= lmax) g[q] = cadd(g[q], fd[q]);
+      if (mc0 <= lmax) g[q] = cadd(g[q], cconj(fc[q]));
+    }
+    return;
+  }
+  for (int m0 = k; m0 <= lmax; m0 += 4 * n) {
+    cplx<T> f[4][NQ];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const cplx<T> *a = Fu + int64_t(min(m0 + u * n, lmax)) * G;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) f[u][q] = a[min(q, nsh - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (m0 + u * n <= lmax)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (q < nsh) g[q] = cadd(g[q], f[u][q]);
+  }
+  for (int m0 = mc0; m0 <= lmax; m0 += 4 * n) {
+    cplx<T> f[4][NQ];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const cplx<T> *a = Fu + int64_t(min(m0 + u * n, lmax)) * G;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) f[u][q] = a[min(q, nsh - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (m0 + u * n <= lmax)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+          if (q < nsh) g[q] = cadd(g[q], cconj(f[u][q]));
+  }
+}
+
+// fold_shells_coop (small rings, n <= NT / 2): Q = NT / n threads per bin,
+// thread (k, q) sums the chunks j = q, q + Q, ... of m (direct m = j n + k,
+// conjugate m = (j + 1) n - k); the partial sums meet in LDS and are added in
+// q order.  Result: bin k of shell q at X[q n + k].  X must hold 2 NT NQ
+// entries; every thread of the workgroup takes part.
+template <typename T, int NT, int NQ>
+__device__ __forceinline__ void fold_shells_coop(const cplx<T> *__restrict__ Fu, int G, int nsh,
+                                                 int n, int lmax, cplx<T> *X) {
+  const int t = threadIdx.x;
+  const int Q = NT / n, nitems = n * Q;  // nitems <= NT
+  const int jmax = lmax / n;
+  cplx<T> d[NQ], c[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) d[q] = c[q] = cplx<T>{T(0), T(0)};
+  if (t < nitems) {
+    const int qq = t / n, k = t - qq * n;
+    for (int j0 = qq; j0 <= jmax; j0 += 4 * Q) {
+      cplx<T> fd[4][NQ], fc[4][NQ];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * Q;
+        const cplx<T> *a = Fu + int64_t(min(j * n + k, lmax)) * G;
+        const cplx<T> *b = Fu + int64_t(min((j + 1) * n - k, lmax)) * G;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          fd[u][q] = a[min(q, nsh - 1)];
+          fc[u][q] = b[min(q, nsh - 1)];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * Q;
+        if (j * n + k <= lmax)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+            if (q < nsh) d[q] = cadd(d[q], fd[u][q]);
+        if ((j + 1) * n - k <= lmax)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+            if (q < nsh) c[q] = cadd(c[q], cconj(fc[u][q]));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      X[(2 * q) * NT + t] = d[q];
+      X[(2 * q + 1) * NT + t] = c[q];
+    }
+  }
+  __syncthreads();
+  cplx<T> g[NQ];
+  if (t < n) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      cplx<T> sd = {T(0), T(0)}, sc = {T(0), T(0)};
+      for (int j = 0; j < Q; ++j) sd = cadd(sd, X[(2 * q) * NT + j * n + t]);
+      for (int j = 0; j < Q; ++j) sc = cadd(sc, X[(2 * q + 1) * NT + j * n + t]);
+      g[q] = cadd(sd, sc);
+    }
+  }
+  __syncthreads();
+  if (t < n) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) X[q * n + t] = g[q];
+  }
+}
+
 template <typename T, int NT, int MAXC, int P>
 __global__ __launch_bounds__(NT) void ring_synth_kernel(
     int lmax, int nrings, const int64_t *__restrict__ ring_start,
@@ -305,6 +429,11 @@ __global__ __launch_bounds__(NT) void ring_synth_kernel(
   int nsh = S - sbase;
   if (nsh > 2 * P) nsh = 2 * P;
 
+  // fold of the ring's coefficients into its n bins (see fold_shells_*):
+  // small rings cooperatively through LDS, large rings per thread
+  const bool coop = n <= NT / 2;
+  if (coop) fold_shells_coop<T, NT, 2 * P>(Fu, G, nsh, n, lmax, X);
+  __syncthreads();
   cplx<T> H[MAXC][4][P];
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
@@ -317,18 +446,11 @@ __global__ __launch_bounds__(NT) void ring_synth_kernel(
       for (int q = 0; q < 2 * P; ++q) g[q] = cplx<T>{T(0), T(0)};
       if (k1 < r) {
         const int k = k1 + r * k2;
-        // G_k = sum_{m = k mod n} F_m + sum_{m = -k mod n, m > 0} conj(F_m)
-        for (int m = k; m <= lmax; m += n) {
-          const cplx<T> *f = Fu + int64_t(m) * G;
+        if (coop) {
 #pragma unroll
-          for (int q = 0; q < 2 * P; ++q)
-            if (q < nsh) g[q] = cadd(g[q], f[q]);
-        }
-        for (int m = (k == 0 ? n : n - k); m <= lmax; m += n) {
-          const cplx<T> *f = Fu + int64_t(m) * G;
-#pragma unroll
-          for (int q = 0; q < 2 * P; ++q)
-            if (q < nsh) g[q] = cadd(g[q], cconj(f[q]));
+          for (int q = 0; q < 2 * P; ++q) g[q] = X[q * n + k];
+        } else {
+          fold_shells_own<T, 2 * P>(Fu, G, nsh, k, n, lmax, g);
         }
       }
 #pragma unroll
@@ -349,6 +471,7 @@ __global__ __launch_bounds__(NT) void ring_synth_kernel(
       H[i][3][p] = cmul(csub(d02, d13), w3);
     }
   }
+  __syncthreads();  // the bins are read before the sub-transforms reuse X
   // ---- four length-r inverse DFTs per pair
 #pragma unroll
   for (int j2 = 0; j2 < 4; ++j2)
@@ -593,6 +716,108 @@ __device__ void fused_fft(cplx<float> *X, int ldx, int A, int N, const float2 *_
   }
 }
 
+// The aliasing fold of one ring's coefficients into its n = 4 r bins, for a
+// unit of two shells (h = G(s) + i G(s + 1)), written to the bin slots of X
+// (bin k = k1 + r k2 at X[k2 ldx + pad(pos(k1))], pos = bit-reversed k1 on
+// lbits bits, or k1 when lbits = 0).  Work item (k, q): the chunks j = q,
+// q + Q, ... of m, each holding the direct term m = j n + k and the conjugate
+// term m = (j + 1) n - k.  Small rings (n <= NT / 2) get Q = NT / n threads per
+// bin, whose partial sums meet in LDS and are added in q order; a bin's direct
+// terms are summed before its conjugate terms in both cases.  The loads of four
+// chunks (or, when every bin has one chunk, of four bins) are issued together
+// at clamped addresses, so a thread never waits on one load at a time.
+template <int NT>
+__device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, int G, bool two,
+                                          int n, int r, int lmax, int lbits, int ldx,
+                                          cplx<float> *X) {
+  const int t = threadIdx.x;
+  const int Q = n <= NT / 2 ? NT / n : 1;
+  const int jmax = lmax / n;  // chunks 0..jmax can hold terms
+  auto ld = [&](int m, cplx<float> &a, cplx<float> &b) {
+    const cplx<float> *f = Fu + int64_t(m) * G;
+    a = f[0];
+    b = two ? f[1] : cplx<float>{0.0f, 0.0f};
+  };
+  auto slot = [&](int k) {
+    const int k2 = (k >= r) + (k >= 2 * r) + (k >= 3 * r);
+    const int k1 = k - k2 * r;
+    const int pos = lbits ? int(__brev(unsigned(k1)) >> (32 - lbits)) : k1;
+    return k2 * ldx + pad(pos);
+  };
+  auto pack = [](cplx<float> g0, cplx<float> g1) {
+    return cplx<float>{g0.re - g1.im, g0.im + g1.re};
+  };
+  if (jmax == 0) {
+    // n > lmax: a bin has at most one direct and one conjugate term
+#pragma unroll 4
+    for (int k = t; k < n; k += NT) {
+      cplx<float> d0, d1, c0, c1;
+      const int mc = n - k;
+      ld(min(k, lmax), d0, d1);
+      ld(min(mc, lmax), c0, c1);
+      cplx<float> g0 = {0.0f, 0.0f}, g1 = {0.0f, 0.0f};
+      if (k <= lmax) {
+        g0 = d0;
+        g1 = d1;
+      }
+      if (mc <= lmax) {
+        g0 = cadd(g0, cconj(c0));
+        g1 = cadd(g1, cconj(c1));
+      }
+      X[slot(k)] = pack(g0, g1);
+    }
+    return;
+  }
+  const int nitems = n * Q;
+  cplx<float> hd = {0.0f, 0.0f}, hc = {0.0f, 0.0f};  // the one item of a Q > 1 thread
+  for (int it = t; it < nitems; it += NT) {
+    const int q = it / n, k = it - q * n;
+    cplx<float> d0 = {0.0f, 0.0f}, d1 = d0, c0 = d0, c1 = d0;
+    for (int j0 = q; j0 <= jmax; j0 += 4 * Q) {
+      cplx<float> fd[4][2], fc[4][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * Q;
+        ld(min(j * n + k, lmax), fd[u][0], fd[u][1]);
+        ld(min((j + 1) * n - k, lmax), fc[u][0], fc[u][1]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u * Q;
+        if (j * n + k <= lmax) {
+          d0 = cadd(d0, fd[u][0]);
+          d1 = cadd(d1, fd[u][1]);
+        }
+        if ((j + 1) * n - k <= lmax) {
+          c0 = cadd(c0, cconj(fc[u][0]));
+          c1 = cadd(c1, cconj(fc[u][1]));
+        }
+      }
+    }
+    if (Q == 1) {
+      X[slot(k)] = pack(cadd(d0, c0), cadd(d1, c1));
+    } else {
+      hd = pack(d0, d1);
+      hc = pack(c0, c1);
+    }
+  }
+  if (Q > 1) {
+    // partial sums: direct at X[it], conjugate at X[nitems + it] (nitems <= NT)
+    if (t < nitems) {
+      X[t] = hd;
+      X[nitems + t] = hc;
+    }
+    __syncthreads();
+    cplx<float> sd = {0.0f, 0.0f}, sc = {0.0f, 0.0f};
+    if (t < n) {
+      for (int q = 0; q < Q; ++q) sd = cadd(sd, X[q * n + t]);
+      for (int q = 0; q < Q; ++q) sc = cadd(sc, X[nitems + q * n + t]);
+    }
+    __syncthreads();
+    if (t < n) X[slot(t)] = cadd(sd, sc);
+  }
+}
+
 // grid: 8 * ceil(nlist / 8) * nunit blocks (XCD-aware as decode_ring_block),
 // one unit = one complex transform (two shells); rings taken from ring_list;
 // BLU selects the Bluestein form.  All four sub-transforms run together on
@@ -621,28 +846,23 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
   const int L = BLU ? blu_len(r) : r;
   const int lbits = ilog2(L);
 
-  // ---- fold, radix-4 split (thread owns k1 = t + i NT), sub-transform inputs -> LDS
+  // ---- fold into the n bins, h_k = G_k(shell s) + i G_k(shell s + 1), with
+  //   G_k = sum_{m = k mod n} F_m + sum_{m = -k mod n, m > 0} conj(F_m)
+  // (m <= lmax).  Bin k = k1 + r k2 is kept at X[k2 ldx + pad(pos(k1))], the
+  // slot the radix-4 split below reads and rewrites (pos = bit-reversed k1 for
+  // the DIT passes of a power-of-two r, k1 for Bluestein).
+  fold_bins<NT>(Fu, G, two, n, r, lmax, BLU ? 0 : lbits, ldx, X);
+  __syncthreads();
+
+  // ---- radix-4 split (thread owns k1 = t + i NT), sub-transform inputs in place
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     const int k1 = t + i * NT;
     if (k1 < r) {
+      const int pos = BLU ? k1 : (lbits ? int(__brev(unsigned(k1)) >> (32 - lbits)) : 0);
       cplx<float> h[4];
 #pragma unroll
-      for (int k2 = 0; k2 < 4; ++k2) {
-        cplx<float> g0 = {0.0f, 0.0f}, g1 = {0.0f, 0.0f};
-        const int k = k1 + r * k2;
-        for (int m = k; m <= lmax; m += n) {
-          const cplx<float> *f = Fu + int64_t(m) * G;
-          g0 = cadd(g0, f[0]);
-          if (two) g1 = cadd(g1, f[1]);
-        }
-        for (int m = (k == 0 ? n : n - k); m <= lmax; m += n) {
-          const cplx<float> *f = Fu + int64_t(m) * G;
-          g0 = cadd(g0, cconj(f[0]));
-          if (two) g1 = cadd(g1, cconj(f[1]));
-        }
-        h[k2] = cplx<float>{g0.re - g1.im, g0.im + g1.re};
-      }
+      for (int k2 = 0; k2 < 4; ++k2) h[k2] = X[k2 * ldx + pad(pos)];
       const cplx<float> w1 = expi_q(k1, n, 0.0f);
       const cplx<float> w2 = cmul(w1, w1);
       const cplx<float> w3 = cmul(w2, w1);
@@ -655,8 +875,6 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q) y[q] = cmul(y[q], cplx<float>{c.x, c.y});
       }
-      // power-of-two r: inputs in bit-reversed order for the DIT passes
-      const int pos = BLU ? k1 : (lbits ? int(__brev(unsigned(k1)) >> (32 - lbits)) : 0);
 #pragma unroll
       for (int q = 0; q < 4; ++q) X[q * ldx + pad(pos)] = y[q];
     }
@@ -722,7 +940,8 @@ int launch_synth2(const ga_plan *p, const int32_t *list, int nlist, int N, int S
                   const float *F, float *maps, const ga_epilogue *epi, hipStream_t stream) {
   if (nlist == 0) return GA_OK;
   const int ldx = padded_len(N);
-  const size_t lds = size_t(4) * ldx * sizeof(cplx<float>);
+  // four sub-transform arrays; the fold's partial sums need 2 NT entries
+  const size_t lds = size_t(std::max(4 * ldx, 2 * NT)) * sizeof(cplx<float>);
   GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds);
   const int nunit = (S + 1) / 2;
   const int nblk = 8 * ((nlist + 7) / 8) * nunit;
@@ -753,6 +972,8 @@ int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float
   if (p->n_pow2 > 0) {
     const int32_t *l = p->d_rings_pow2;
     const int n = p->n_pow2;
+    // LDS per workgroup: 4 arrays of N padded complex floats (67.6 KB at N =
+    // 2048, 135 KB at 4096: one workgroup per CU, so 512 threads there)
     if (Np <= 256) rc = launch_synth2<256, 1, false>(p, l, n, Np, S, G, F, maps, epi, stream);
     else if (Np <= 512) rc = launch_synth2<256, 2, false>(p, l, n, Np, S, G, F, maps, epi, stream);
     else if (Np <= 1024) rc = launch_synth2<256, 4, false>(p, l, n, Np, S, G, F, maps, epi, stream);
@@ -763,10 +984,13 @@ int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float
   if (p->n_blu > 0) {
     const int32_t *l = p->d_rings_blu;
     const int n = p->n_blu;
-    if (rb <= 256) rc = launch_synth2<256, 1, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
-    else if (rb <= 512) rc = launch_synth2<256, 2, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
-    else if (rb <= 1024) rc = launch_synth2<256, 4, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
-    else rc = launch_synth2<256, 8, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+    if (Nb <= 2048) {
+      if (rb <= 256) rc = launch_synth2<256, 1, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+      else if (rb <= 512) rc = launch_synth2<256, 2, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+      else rc = launch_synth2<256, 4, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+    } else {
+      rc = launch_synth2<512, 4, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+    }
   }
   return rc;
 }
@@ -818,11 +1042,13 @@ int launch_ring(const ga_plan *p, int S, int G, const void *in, void *out, const
   GA_REQUIRE(G == S || G % (2 * P) == 0, GA_ERR_INVALID, "shell group %d incompatible", G);
   const int nunit = (S + 2 * P - 1) / (2 * P);
   const int nblk = 8 * ((p->nrings + 7) / 8) * nunit;
-  const size_t lds = size_t(P) * ldx * 2 * sizeof(T);
-  GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds);
+  // P work arrays; the cooperative fold of small rings needs 4 P NT entries
+  const size_t lds0 = size_t(P) * ldx * 2 * sizeof(T);
+  GA_REQUIRE(lds0 <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds0);
 #define GA_SYNTH_LAUNCH(NT_, MAXC_, P_)                                                        \
   do {                                                                                        \
     auto kfn = ring_synth_kernel<T, NT_, MAXC_, P_>;                                          \
+    const size_t lds = std::max(lds0, size_t(4) * P_ * NT_ * 2 * sizeof(T));                  \
     GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn,                                       \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
     hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,           \

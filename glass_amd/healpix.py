@@ -231,15 +231,17 @@ def alm2map_spin(alms, nside: int, spin: int, lmax: int, precision: str = "fp64"
 
 
 def spin_batch_f32(E, B, nside: int, spin: int, lmax: int):
-    """E, B: [nalm, S] complex64 device tensors -> [S, 2, npix] float32 (Q, U)."""
+    """E, B: [nalm, S] complex64 device tensors -> [S, 2, npix] float32 (Q, U);
+    B = None means B = 0."""
     T = _lib.torch()
     S = int(E.shape[1])
     plan = _lib.get_plan(nside, lmax)
     plan.prepare_spin(int(spin))
     ws = _lib.workspace(plan.workspace_bytes(2 * S, 4))
     out = T.empty((S, 2, nside2npix(nside)), dtype=T.float32, device="cuda")
-    _lib.call("ga_alm2map_spin_batch_f32", plan.h, int(spin), S, _lib.ptr(E), _lib.ptr(B),
-              _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    _lib.call("ga_alm2map_spin_batch_f32", plan.h, int(spin), S, _lib.ptr(E),
+              None if B is None else _lib.ptr(B), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+              _lib.stream_ptr())
     return out
 
 

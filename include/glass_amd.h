@@ -56,8 +56,12 @@ enum ga_tune_key {
   GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2 or 0 = automatic */
   GA_TUNE_MFMA_CHUNK = 2, /* FP32 path: quads per accumulation chunk, 0 = none */
   GA_TUNE_MFMA_PATH = 3,  /* 0 = automatic (split FP16), 1 = FP32 MFMA, 2 = split FP16 */
-  GA_TUNE_RING_PATH = 4   /* FP32 ring synthesis: 1 = radix-4 LDS stages (default),
+  GA_TUNE_RING_PATH = 4,  /* FP32 ring synthesis: 1 = radix-4 LDS stages (default),
                              2 = fused radix-2^K passes, 0 = automatic (= 2) */
+  GA_TUNE_SPIN_PATH = 5,  /* FP32 spin synthesis: 0 = automatic (split FP16 MFMA),
+                             1 = VALU, 2 = split FP16 MFMA */
+  GA_TUNE_ANAL_PATH = 6   /* FP32 analysis Legendre stage: 0 = automatic (split FP16
+                             MFMA), 1 = VALU, 2 = split FP16 MFMA */
 };
 int ga_set_tuning(int32_t key, int32_t value);
 int ga_get_tuning(int32_t key, int32_t *value);
@@ -142,8 +146,9 @@ int ga_map2alm_batch_f32(const ga_plan *plan, int32_t nshell, const float *maps,
 /* Spin-s synthesis (s = 1, 2) of [E, B] into [Q, U], HEALPix convention
  * Q + iU = -sum (E + iB) sY_lm.  glass/healpix.py:81-108 alm2map_spin, called
  * at glass/lensing.py:343 (s = 1) and :366, :428 (s = 2).
- * f64: one map pair, maps = [Q | U] (2 npix).  f32: S pairs [S][2][npix].
- * workspace: ga_workspace_size for 2 S maps. */
+ * f64: one map pair, maps = [Q | U] (2 npix).  f32: S pairs [S][2][npix];
+ * almB may be NULL there (B = 0, the lensing case: one contraction instead of
+ * two).  workspace: ga_workspace_size for 2 S maps. */
 int ga_alm2map_spin_f64(const ga_plan *plan, int32_t spin, const double *almE,
                         const double *almB, double *maps, void *workspace, size_t ws_bytes,
                         void *stream);

@@ -137,3 +137,77 @@ def test_ring_paths_agree(nside, lmax):
     for s in (0, S - 1):
         ref = eng.fp64_reference_map(s).cpu().numpy()
         assert relerr(out[2][s].cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+@pytest.mark.parametrize("nside,lmax,S", [(8, 20, 3), (64, 150, 5), (256, 511, 4)])
+def test_spin_mfma_matches_valu_and_fp64(spin, nside, lmax, S):
+    """FP32 spin synthesis: the split-FP16 MFMA path (E only, and E + B in two
+    passes) against the VALU path and the FP64 path, max |error| / RMS < 1e-5;
+    B = NULL gives the same maps as an explicit zero B."""
+    from glass_amd import _lib
+    from glass_amd.healpix import spin_batch_f32
+
+    print(f"spin mfma spin={spin} nside={nside} lmax={lmax} S={S}", flush=True)
+    rng = np.random.default_rng(spin * 7 + nside)
+    n = O.nalm(lmax)
+    ell = np.concatenate([np.arange(m, lmax + 1) for m in range(lmax + 1)]).astype(float)
+    amp = np.sqrt(1e-3 / (ell + 1.0) ** 1.5)
+    E = (rng.standard_normal((n, S)) + 1j * rng.standard_normal((n, S))) * amp[:, None]
+    B = (rng.standard_normal((n, S)) + 1j * rng.standard_normal((n, S))) * amp[:, None]
+    Ed = torch.as_tensor(E).to(torch.complex64).cuda().contiguous()
+    Bd = torch.as_tensor(B).to(torch.complex64).cuda().contiguous()
+    v = _lib.ctypes.c_int32()
+    _lib.call("ga_get_tuning", _lib.TUNE_SPIN_PATH, _lib.ctypes.byref(v))
+    out = {}
+    try:
+        for path in (1, 2):
+            _lib.call("ga_set_tuning", _lib.TUNE_SPIN_PATH, path)
+            out[path, "eb"] = spin_batch_f32(Ed, Bd, nside, spin, lmax).cpu().numpy()
+            out[path, "e0"] = spin_batch_f32(Ed, torch.zeros_like(Ed), nside, spin, lmax).cpu().numpy()
+        out[2, "none"] = spin_batch_f32(Ed, None, nside, spin, lmax).cpu().numpy()
+    finally:
+        _lib.call("ga_set_tuning", _lib.TUNE_SPIN_PATH, v.value)
+    assert np.array_equal(out[2, "none"], out[2, "e0"]) or \
+        relerr(out[2, "none"], out[2, "e0"]) < 1e-6
+    for kind in ("eb", "e0"):
+        for s in range(S):
+            for c in range(2):
+                assert relerr(out[2, kind][s, c], out[1, kind][s, c]) < 1e-5, (kind, s, c)
+    for s in (0, S - 1):
+        q, u = hp.alm2map_spin([E[:, s], B[:, s]], nside, spin, lmax)
+        assert relerr(out[2, "eb"][s, 0], q) < 1e-5
+        assert relerr(out[2, "eb"][s, 1], u) < 1e-5
+        q, u = hp.alm2map_spin([E[:, s], np.zeros_like(E[:, s])], nside, spin, lmax)
+        assert relerr(out[2, "none"][s, 0], q) < 1e-5
+        assert relerr(out[2, "none"][s, 1], u) < 1e-5
+
+
+@pytest.mark.parametrize("nside,lmax,S,niter", [(8, 20, 3, 0), (64, 150, 7, 0), (256, 600, 5, 1),
+                                                (512, 1023, 17, 0)])
+def test_analysis_mfma_matches_valu_and_fp64(nside, lmax, S, niter):
+    """FP32 batched analysis: the split-FP16 MFMA contraction against the VALU
+    one and against the FP64 path (max |error| / RMS of the alm < 1e-5)."""
+    from glass_amd import _lib
+
+    print(f"analysis mfma nside={nside} lmax={lmax} S={S} niter={niter}", flush=True)
+    rng = np.random.default_rng(nside + S)
+    maps = torch.as_tensor(rng.standard_normal((S, 12 * nside * nside))).cuda()
+    m32 = maps.float().contiguous()
+    v = _lib.ctypes.c_int32()
+    _lib.call("ga_get_tuning", _lib.TUNE_ANAL_PATH, _lib.ctypes.byref(v))
+    out = {}
+    try:
+        for path in (1, 2):
+            _lib.call("ga_set_tuning", _lib.TUNE_ANAL_PATH, path)
+            out[path] = hp.map2alm_batch_f32(m32, lmax, niter).T.cpu().numpy()
+    finally:
+        _lib.call("ga_set_tuning", _lib.TUNE_ANAL_PATH, v.value)
+    for s in range(S):
+        a1, a2 = out[1][s], out[2][s]
+        scale = np.sqrt(np.mean(np.abs(a1) ** 2))
+        assert np.max(np.abs(a2 - a1)) / scale < 1e-5, s
+    for s in (0, S - 1):
+        ref = hp.map2alm(maps[s], lmax=lmax, iter=niter, pol=False).cpu().numpy()
+        scale = np.sqrt(np.mean(np.abs(ref) ** 2))
+        assert np.max(np.abs(out[2][s] - ref)) / scale < 1e-5, s
