@@ -48,7 +48,7 @@ struct ga_tuning {
   int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto)
   int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
   int mfma_path = 0;    // Legendre stage: 0 auto (= split FP16), 1 FP32 MFMA, 2 split FP16
-  int ring_path = 1;    // FP32 ring synthesis: 0 auto (= fused passes), 1 radix-4 stages, 2 fused passes
+  int ring_path = 0;    // FP32 ring synthesis: 0 auto (= fused passes), 1 radix-4 stages, 2 fused passes
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
 };
