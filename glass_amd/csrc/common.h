@@ -45,7 +45,7 @@ void clear_error();
 
 // ---- tuning knobs (set through ga_set_tuning; read by the launchers) ----
 struct ga_tuning {
-  int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto)
+  int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto; 4 = wave pairs)
   int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
   int mfma_path = 0;    // Legendre stage: 0 auto (= split FP16), 1 FP32 MFMA, 2 split FP16
   int ring_path = 0;    // FP32 ring synthesis: 0 auto (= fused passes), 1 radix-4 stages, 2 fused passes

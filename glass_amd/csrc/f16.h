@@ -50,44 +50,43 @@ __device__ __forceinline__ void split_pair(float a, float b, uint32_t &hi, uint3
   lo = bits_h2(l);
 }
 
-// The MFMAs of one 32-row slab.  ph / pl [parity][group] hold a lane's 32
-// values of the slab as packed FP16 pairs (group g: k = 2g, 2g + 1 of the 16
-// rows of that parity); v_permlane32_swap(X, Y) = ([X_lo | Y_lo], [X_hi | Y_hi])
-// turns them into the A operands of two row tiles: tile 0 = the rows computed by
-// lanes 0..31, tile 1 = those of lanes 32..63.  bh / bl [parity][column tile]
-// are the pre-swizzled B operands.  acc [parity][row tile][column tile].  The
-// three products are interleaved so that dependent MFMAs are 4 NT - 1
-// instructions apart.
-template <int NT>
-__device__ __forceinline__ void slab_mfma(const uint32_t (&ph)[2][8], const uint32_t (&pl)[2][8],
-                                          const uint4 (&bh)[2][NT], const uint4 (&bl)[2][NT],
-                                          f32x16 (&acc)[2][2][NT]) {
-  uint32_t ah[2][2][4], al[2][2][4];
+// A operands of one 32-row slab from a lane's packed values.  ph / pl
+// [parity][group] hold the lane's 32 values of the slab as packed FP16 pairs
+// (group g: k = 2g, 2g + 1 of the 16 rows of that parity);
+// v_permlane32_swap(X, Y) = ([X_lo | Y_lo], [X_hi | Y_hi]) turns them into
+// the A operands of two row tiles: tile 0 = the rows computed by lanes 0..31,
+// tile 1 = those of lanes 32..63.  ah / al [parity][row tile].
+__device__ __forceinline__ void slab_a(const uint32_t (&ph)[2][8], const uint32_t (&pl)[2][8],
+                                       uint4 (&ah)[2][2], uint4 (&al)[2][2]) {
+  uint32_t h[2][2][4], l[2][2][4];
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       const auto sh = __builtin_amdgcn_permlane32_swap(ph[p][w], ph[p][w + 4], false, false);
       const auto sl = __builtin_amdgcn_permlane32_swap(pl[p][w], pl[p][w + 4], false, false);
-      ah[p][0][w] = sh[0];
-      ah[p][1][w] = sh[1];
-      al[p][0][w] = sl[0];
-      al[p][1][w] = sl[1];
+      h[p][0][w] = sh[0];
+      h[p][1][w] = sh[1];
+      l[p][0][w] = sl[0];
+      l[p][1][w] = sl[1];
     }
-  f16x8 Ah[2][2], Al[2][2], Bh[2][NT], Bl[2][NT];
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      Ah[p][t] = __builtin_bit_cast(f16x8, uint4{ah[p][t][0], ah[p][t][1], ah[p][t][2], ah[p][t][3]});
-      Al[p][t] = __builtin_bit_cast(f16x8, uint4{al[p][t][0], al[p][t][1], al[p][t][2], al[p][t][3]});
+      ah[p][t] = uint4{h[p][t][0], h[p][t][1], h[p][t][2], h[p][t][3]};
+      al[p][t] = uint4{l[p][t][0], l[p][t][1], l[p][t][2], l[p][t][3]};
     }
-#pragma unroll
-    for (int ct = 0; ct < NT; ++ct) {
-      Bh[p][ct] = __builtin_bit_cast(f16x8, bh[p][ct]);
-      Bl[p][ct] = __builtin_bit_cast(f16x8, bl[p][ct]);
-    }
-  }
+}
+
+// The MFMAs of one slab: acc [parity][row tile][column tile] += A B with the
+// three split products; bh / bl [parity][column tile] are the pre-swizzled B
+// operands.  Accumulators are interleaved so that dependent MFMAs are
+// 4 NT - 1 instructions apart.
+template <int NT>
+__device__ __forceinline__ void slab_mfma_a(const uint4 (&ah)[2][2], const uint4 (&al)[2][2],
+                                            const uint4 (&bh)[2][NT], const uint4 (&bl)[2][NT],
+                                            f32x16 (&acc)[2][2][NT]) {
 #pragma unroll
   for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -96,10 +95,20 @@ __device__ __forceinline__ void slab_mfma(const uint32_t (&ph)[2][8], const uint
       for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
-          const f16x8 a = (q == 2) ? Al[p][t] : Ah[p][t];
-          const f16x8 bb = (q == 1) ? Bl[p][ct] : Bh[p][ct];
+          const f16x8 a = __builtin_bit_cast(f16x8, (q == 2) ? al[p][t] : ah[p][t]);
+          const f16x8 bb = __builtin_bit_cast(f16x8, (q == 1) ? bl[p][ct] : bh[p][ct]);
           acc[p][t][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb, acc[p][t][ct], 0, 0, 0);
         }
+}
+
+// both steps
+template <int NT>
+__device__ __forceinline__ void slab_mfma(const uint32_t (&ph)[2][8], const uint32_t (&pl)[2][8],
+                                          const uint4 (&bh)[2][NT], const uint4 (&bl)[2][NT],
+                                          f32x16 (&acc)[2][2][NT]) {
+  uint4 ah[2][2], al[2][2];
+  slab_a(ph, pl, ah, al);
+  slab_mfma_a<NT>(ah, al, bh, bl, acc);
 }
 
 }  // namespace f16

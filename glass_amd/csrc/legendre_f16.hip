@@ -150,6 +150,147 @@ __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int 
   B[unit * 128 + 64 + L] = uint4{lv[0], lv[1], lv[2], lv[3]};
 }
 
+// One slab (32 rows from l0) of a lane's recurrence: lambda 2^8 for its ring
+// pair, split and packed as FP16 pairs (ph / pl [parity][group], group g =
+// rows k = 2g, 2g + 1 of each parity), with the scaled-seed bookkeeping of
+// the turn-on region.  cb: the slab's (a, a b) coefficient pairs in LDS.
+__device__ __forceinline__ void slab_values(const double2 *__restrict__ cb, double x, int l0,
+                                            int lmax, double &p0, double &p1, int &k,
+                                            uint32_t (&ph)[2][8], uint32_t (&pl)[2][8]) {
+  const bool slow = (l0 + kSlab - 1 > lmax) || wave_any(k < 0);
+  if (!slow) {
+    // every lane in range and a full slab: the plain recurrence.  The
+    // coefficients of group g + 1 are read from LDS while group g runs; the
+    // scheduling barrier keeps later groups' reads from being hoisted
+    // further (each would hold 8 VGPRs for the rest of the slab).
+    double2 cc[4], cn[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cc[j] = cb[j];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      if (g < 7) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cn[j] = cb[4 * (g + 1) + j];
+      }
+      float v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double pn = fma(cc[j].x * x, p1, -(cc[j].y * p0));
+        p0 = p1;
+        p1 = pn;
+        v[j] = float(pn);
+      }
+      split_pair(v[0], v[2], ph[0][g], pl[0][g]);
+      split_pair(v[1], v[3], ph[1][g], pl[1][g]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cc[j] = cn[j];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const int lg = l0 + 4 * g;
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      bool dead = true;  // every lane still scaled: the group's values are all zero
+      if (lg <= lmax) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double2 c = cb[4 * g + j];
+          const double pn = fma(c.x * x, p1, -(c.y * p0));
+          p0 = p1;
+          p1 = pn;
+          v[j] = p1;
+        }
+        // k is fixed within a group: a scaled lane's values are < 2^-290
+        // (zero in FP32); rescale once per group (4 steps grow |p| by far
+        // less than the 2^700 of headroom above the threshold)
+        dead = wave_all(k < 0);
+        if (k != 0) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[j] = 0.0;
+        }
+        if (k < 0 && fabs(p1) > 0x1p300) {
+          p1 *= 0x1p-600;
+          p0 *= 0x1p-600;
+          ++k;
+        }
+        if (lg + 3 > lmax) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (lg + j > lmax) v[j] = 0.0;
+        }
+      }
+      if (dead) {
+        ph[0][g] = ph[1][g] = pl[0][g] = pl[1][g] = 0u;
+      } else {
+        split_pair(float(v[0]), float(v[2]), ph[0][g], pl[0][g]);
+        split_pair(float(v[1]), float(v[3]), ph[1][g], pl[1][g]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Epilogue of a wave's accumulators (as in the FP32 stage): D layout of
+// 32x32: lane L, element e -> row 8 (e/4) + 4 (L/32) + e%4, column L % 32.
+// Unscale by 2^-(8 + shift of the column), north = E + O, south = E - O,
+// times the ring phase e^{i m phi0} (each lane computes its own ring's phase
+// in FP64; rows fetch it by shuffle; the (re, im) columns of a shell sit in
+// lanes (2c, 2c + 1)).  Ring buffer layout: real column c of ring r at
+//   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8).
+template <int NT>
+__device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m, int lmax,
+                                           int npairs, int nrings, int ncols, int colbase, int rp0,
+                                           int rpc, int lane, const int32_t *__restrict__ ring_pnum,
+                                           const int32_t *__restrict__ ring_pden,
+                                           const int32_t *__restrict__ colexp,
+                                           float *__restrict__ F) {
+  const int half = lane >> 5, col_in = lane & 31;
+  float phc, phs;
+  {
+    const int pnum = ring_pnum[rpc], pden = ring_pden[rpc];
+    double ps = 0.0, pc = 1.0;
+    if (pnum) sincospi(2.0 * double((int64_t(m) * pnum) % pden) / double(pden), &ps, &pc);
+    phc = float(pc);
+    phs = float(ps);
+  }
+  float fac[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) {
+    const int col = colbase + ct * 32 + col_in;
+    fac[ct] = col < ncols ? ldexpf(1.0f, -(kLamExp + colexp[col])) : 0.0f;
+  }
+  const int ngrp = (ncols + 7) / 8;
+  const bool is_im = (col_in & 1) != 0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int row = 32 * t + 8 * (e >> 2) + 4 * half + (e & 3);
+      const float c = __shfl(phc, row, 64), sn = __shfl(phs, row, 64);
+      const int rpo = rp0 + row;
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        const int col = colbase + ct * 32 + col_in;
+        const float ev = acc[0][t][ct][e] * fac[ct], ov = acc[1][t][ct][e] * fac[ct];
+        const float nv = ev + ov, sv = ev - ov;
+        const float no = __shfl_xor(nv, 1, 64), so = __shfl_xor(sv, 1, 64);
+        // re' = re c - im s,  im' = re s + im c
+        const float nrot = is_im ? (no * sn + nv * c) : (nv * c - no * sn);
+        const float srot = is_im ? (so * sn + sv * c) : (sv * c - so * sn);
+        if (rpo < npairs && col < ncols) {
+          const float nrz = (m == 0 && is_im) ? 0.0f : nrot;
+          const float srz = (m == 0 && is_im) ? 0.0f : srot;
+          const int64_t gofs = (int64_t(col >> 3) * (lmax + 1) + m) * 8 + (col & 7);
+          F[int64_t(rpo) * ngrp * (lmax + 1) * 8 + gofs] = nrz;
+          if (rpo < npairs - 1)
+            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * 8 + gofs] = srz;
+        }
+      }
+    }
+  }
+}
+
 template <int NT, int MINW>
 __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
     int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
@@ -227,78 +368,7 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
     const double2 *__restrict__ cb = cbuf[wave][cur];
 
     uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
-    const bool slow = (l0 + kSlab - 1 > lmax) || wave_any(k < 0);
-    if (!slow) {
-      // every lane in range and a full slab: the plain recurrence.  The
-      // coefficients of group g + 1 are read from LDS while group g runs; the
-      // scheduling barrier keeps later groups' reads from being hoisted
-      // further (each would hold 8 VGPRs for the rest of the slab).
-      double2 cc[4], cn[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) cc[j] = cb[j];
-#pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        if (g < 7) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) cn[j] = cb[4 * (g + 1) + j];
-        }
-        float v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double pn = fma(cc[j].x * x, p1, -(cc[j].y * p0));
-          p0 = p1;
-          p1 = pn;
-          v[j] = float(pn);
-        }
-        split_pair(v[0], v[2], ph[0][g], pl[0][g]);
-        split_pair(v[1], v[3], ph[1][g], pl[1][g]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cc[j] = cn[j];
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    } else {
-#pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        const int lg = l0 + 4 * g;
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        bool dead = true;  // every lane still scaled: the group's values are all zero
-        if (lg <= lmax) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const double2 c = cb[4 * g + j];
-            const double pn = fma(c.x * x, p1, -(c.y * p0));
-            p0 = p1;
-            p1 = pn;
-            v[j] = p1;
-          }
-          // k is fixed within a group: a scaled lane's values are < 2^-290
-          // (zero in FP32); rescale once per group (4 steps grow |p| by far
-          // less than the 2^700 of headroom above the threshold)
-          dead = wave_all(k < 0);
-          if (k != 0) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = 0.0;
-          }
-          if (k < 0 && fabs(p1) > 0x1p300) {
-            p1 *= 0x1p-600;
-            p0 *= 0x1p-600;
-            ++k;
-          }
-          if (lg + 3 > lmax) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-              if (lg + j > lmax) v[j] = 0.0;
-          }
-        }
-        if (dead) {
-          ph[0][g] = ph[1][g] = pl[0][g] = pl[1][g] = 0u;
-        } else {
-          split_pair(float(v[0]), float(v[2]), ph[0][g], pl[0][g]);
-          split_pair(float(v[1]), float(v[3]), ph[1][g], pl[1][g]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
+    slab_values(cb, x, l0, lmax, p0, p1, k, ph, pl);
     if (lane < kSlab) cbuf[wave][cur ^ 1][lane] = cpre;
     __builtin_amdgcn_wave_barrier();
 
@@ -307,56 +377,127 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
     if (wave_any(k == 0)) f16::slab_mfma<NT>(ph, pl, bh, bl, acc);
   }
 
-  // epilogue (as in the FP32 stage): D layout of 32x32: lane L, element e ->
-  // row 8 (e/4) + 4 (L/32) + e%4, column L % 32.  Unscale by 2^-(8 + shift of
-  // the column), north = E + O, south = E - O, times the ring phase e^{i m phi0}
-  // (each lane computes its own ring's phase in FP64; rows fetch it by
-  // shuffle; the (re, im) columns of a shell sit in lanes (2c, 2c + 1)).
-  // Ring buffer layout: real column c of ring r at
-  //   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8).
-  float phc, phs;
-  {
-    const int pnum = ring_pnum[rpc], pden = ring_pden[rpc];
-    double ps = 0.0, pc = 1.0;
-    if (pnum) sincospi(2.0 * double((int64_t(m) * pnum) % pden) / double(pden), &ps, &pc);
-    phc = float(pc);
-    phs = float(ps);
+  store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
+                 colexp, F);
+}
+
+// Producer / consumer form for wide batches (more than 64 real columns).
+// The accumulators of 64 ring pairs x 64 columns fill a wave's register
+// budget at two waves per SIMD, so with 128 columns the single-wave form runs
+// the FP64 recurrence of every ring pair twice.  Here a workgroup of 8 waves
+// pairs wave w (producer: recurrence, splitting, MFMAs for columns 0..63 of
+// the 128-column group) with wave w + 4 (consumer: MFMAs for columns
+// 64..127) on the same 64 ring pairs; the producer passes each slab's A
+// operands through LDS (double-buffered, one workgroup barrier per slab), so
+// every recurrence value serves 128 columns and the consumer's MFMAs overlap
+// the producer's next slab on the VALU.
+__global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
+    int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
+    const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
+    const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
+    const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
+    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F) {
+  constexpr int NT = 2;
+  // XCD-aware decode as in the single-wave form (ncg counts 128-column groups)
+  const int b = blockIdx.x;
+  const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
+  const int m = 8 * (kq / G) + xcd;
+  if (m > lmax) return;  // whole workgroup
+  const int rem = kq - (kq / G) * G;
+  const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int pr = wave & 3;          // ring-pair group of the pair
+  const bool producer = wave < 4;
+  const int rp0 = rg * 256 + pr * 64;
+  const bool active = rp0 < npairs;  // same for both waves of a pair
+  const int rp = rp0 + lane;
+  const int rpc = rp < npairs ? rp : npairs - 1;
+  const double x = z[rpc];
+  double p0, p1 = 0.0;
+  int k;
+  legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
+
+  const int64_t base = hp_index(lmax, m, m);
+  const double2 *__restrict__ co = ab + base;
+  const int64_t s0 = slab_off[m];
+  const int nsl = int(nslab_of(lmax, m));
+  const int colbase = cg * 128 + (producer ? 0 : 64);
+  int ctile[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) ctile[ct] = min(colbase / 32 + ct, nct - 1);
+
+  f32x16 acc[2][2][NT];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
+
+  // A operands of a slab, [buffer][pair][hi p0t0 p0t1 p1t0 p1t1 | lo ...][lane],
+  // and whether the slab has any nonzero lambda
+  __shared__ __align__(16) uint4 abuf[2][4][8][64];
+  __shared__ int aflag[2][4];
+  __shared__ double2 cbuf[4][2][kSlab];  // the producers' coefficient buffers
+  const int lastc = lmax - m;
+  double2 cpre = double2{0.0, 0.0};
+  if (producer) {
+    cpre = co[min(lane & (kSlab - 1), lastc)];
+    if (lane < kSlab) cbuf[pr][0][lane] = cpre;
+    __builtin_amdgcn_wave_barrier();
   }
-  float fac[NT];
+
+  for (int s = 0; s < nsl; ++s) {
+    const int l0 = m + kSlab * s;
+    const int cur = s & 1;
+    uint4 bh[2][NT], bl[2][NT];
 #pragma unroll
-  for (int ct = 0; ct < NT; ++ct) {
-    const int col = colbase + ct * 32 + col_in;
-    fac[ct] = col < ncols ? ldexpf(1.0f, -(kLamExp + colexp[col])) : 0.0f;
-  }
-  const int ngrp = (ncols + 7) / 8;
-  const bool is_im = (col_in & 1) != 0;
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int row = 32 * t + 8 * (e >> 2) + 4 * half + (e & 3);
-      const float c = __shfl(phc, row, 64), sn = __shfl(phs, row, 64);
-      const int rpo = rp0 + row;
+    for (int p = 0; p < 2; ++p)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) {
-        const int col = colbase + ct * 32 + col_in;
-        const float ev = acc[0][t][ct][e] * fac[ct], ov = acc[1][t][ct][e] * fac[ct];
-        const float nv = ev + ov, sv = ev - ov;
-        const float no = __shfl_xor(nv, 1, 64), so = __shfl_xor(sv, 1, 64);
-        // re' = re c - im s,  im' = re s + im c
-        const float nrot = is_im ? (no * sn + nv * c) : (nv * c - no * sn);
-        const float srot = is_im ? (so * sn + sv * c) : (sv * c - so * sn);
-        if (rpo < npairs && col < ncols) {
-          const float nrz = (m == 0 && is_im) ? 0.0f : nrot;
-          const float srz = (m == 0 && is_im) ? 0.0f : srot;
-          const int64_t gofs = (int64_t(col >> 3) * (lmax + 1) + m) * 8 + (col & 7);
-          F[int64_t(rpo) * ngrp * (lmax + 1) * 8 + gofs] = nrz;
-          if (rpo < npairs - 1)
-            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * 8 + gofs] = srz;
-        }
+        const uint4 *__restrict__ u = B + (((s0 + s) * 2 + p) * nct + ctile[ct]) * 128;
+        bh[p][ct] = u[lane];
+        bl[p][ct] = u[64 + lane];
       }
+    if (producer && active) {
+      cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
+      uint32_t ph[2][8], pl[2][8];
+      slab_values(cbuf[pr][cur], x, l0, lmax, p0, p1, k, ph, pl);
+      if (lane < kSlab) cbuf[pr][cur ^ 1][lane] = cpre;
+      __builtin_amdgcn_wave_barrier();
+      const bool need = wave_any(k == 0);
+      if (need) {
+        uint4 ah[2][2], al[2][2];
+        f16::slab_a(ph, pl, ah, al);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            abuf[cur][pr][2 * p + t][lane] = ah[p][t];
+            abuf[cur][pr][4 + 2 * p + t][lane] = al[p][t];
+          }
+      }
+      if (lane == 0) aflag[cur][pr] = need ? 1 : 0;
+    }
+    __syncthreads();  // slab s's A operands are in LDS; buffer cur ^ 1 is free
+    // both waves of the pair read the A operands back (the producer too: its
+    // registers are then free across the barrier)
+    if (active && aflag[cur][pr] != 0) {
+      uint4 ah[2][2], al[2][2];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          ah[p][t] = abuf[cur][pr][2 * p + t][lane];
+          al[p][t] = abuf[cur][pr][4 + 2 * p + t][lane];
+        }
+      f16::slab_mfma_a<NT>(ah, al, bh, bl, acc);
     }
   }
+
+  if (!active) return;
+  store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
+                 colexp, F);
 }
 
 }  // namespace
@@ -396,7 +537,21 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
   const ga_tuning &tu = tuning();
   int nt = tu.mfma_nt;
-  if (nt != 1 && nt != 2) nt = ncols <= 32 ? 1 : 2;
+  if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : (ncols <= 64 ? 2 : 4);
+  if (nt == 4) {
+    // producer / consumer pairs over 128-column groups
+    const int nrg = (p->npairs + 255) / 256;
+    const int ncg = (ncols + 127) / 128;
+    const int nm8 = (p->lmax + 1 + 7) / 8;
+    const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
+    GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
+    hipLaunchKernelGGL(legendre_synth_f16_pc_kernel, dim3(unsigned(nblk)), dim3(512), 0, stream,
+                       p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, p->d_z, p->d_sth,
+                       p->d_aab, p->d_cm, p->d_ring_pnum, p->d_ring_pden, p->d_slab_off, B, colexp,
+                       F);
+    GA_LAUNCH_CHECK();
+    return GA_OK;
+  }
   const int nrg = (p->npairs + 255) / 256;
   const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
   const int nm8 = (p->lmax + 1 + 7) / 8;

@@ -185,7 +185,8 @@ extern "C" int ga_set_tuning(int32_t key, int32_t value) {
   clear_error();
   switch (key) {
     case GA_TUNE_MFMA_TILES:
-      GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "mfma tiles must be 0, 1 or 2");
+      GA_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4, GA_ERR_INVALID,
+                 "mfma tiles must be 0, 1, 2 or 4");
       tuning().mfma_nt = value;
       return GA_OK;
     case GA_TUNE_MFMA_CHUNK:

@@ -211,3 +211,28 @@ def test_analysis_mfma_matches_valu_and_fp64(nside, lmax, S, niter):
         ref = hp.map2alm(maps[s], lmax=lmax, iter=niter, pol=False).cpu().numpy()
         scale = np.sqrt(np.mean(np.abs(ref) ** 2))
         assert np.max(np.abs(out[2][s] - ref)) / scale < 1e-5, s
+
+
+@pytest.mark.parametrize("nside,lmax,S", [(32, 63, 40), (128, 255, 64), (64, 150, 100)])
+def test_wave_pair_legendre_matches_single_wave(nside, lmax, S):
+    """The producer / consumer wave-pair form of the split-FP16 Legendre stage
+    (tiles = 4) issues the same MFMAs in the same order as the single-wave
+    form with two tiles, so the maps agree bit for bit."""
+    from glass_amd import autotune
+
+    print(f"wave pairs nside={nside} lmax={lmax} S={S}", flush=True)
+    gls = shell_spectra(S, lmax, rho=(0.6, 0.3, 0.1))
+    eng = LognormalShells(gls, nside, ncorr=3, seed=11, lmax=lmax)
+    saved = autotune.current()
+    out = {}
+    try:
+        for nt in (2, 4):
+            autotune.apply({"path": 2, "nt": nt})
+            eng.run()
+            torch.cuda.synchronize()
+            out[nt] = eng.maps.clone()
+    finally:
+        autotune.apply(saved)
+    assert torch.equal(out[2], out[4])
+    ref = eng.fp64_reference_map(S - 1).cpu().numpy()
+    assert relerr(out[4][S - 1].cpu().numpy(), ref) < 1e-5
