@@ -2,8 +2,7 @@
 
 For one geometry and batch width it times the candidate kernel
 configurations -- the split-FP16 path with one or two 32-column tiles per
-wave or with producer / consumer wave pairs, and the FP32 MFMA path -- on the
-engine's own alm (HIP events on the
+wave, and the FP32 MFMA path -- on the engine's own alm (HIP events on the
 launch stream, median of ``reps``), keeps the fastest, and caches the choice
 per (nside, lmax, shells) in a JSON file (``GLASS_AMD_TUNE_CACHE``, default
 ``~/.cache/glass_amd/legendre_tune.json``).  The choice is applied through the
@@ -21,9 +20,8 @@ import numpy as np
 
 from . import _lib
 
-# (path, tiles): path 2 = split FP16, 1 = FP32 MFMA; tiles 4 = producer /
-# consumer wave pairs over 128 columns (split FP16)
-CANDIDATES = ((2, 1), (2, 2), (2, 4), (1, 2))
+# (path, tiles): path 2 = split FP16, 1 = FP32 MFMA
+CANDIDATES = ((2, 1), (2, 2), (1, 2))
 
 
 def cache_path() -> pathlib.Path:

@@ -47,6 +47,7 @@ __device__ __forceinline__ int anal_shift(float mx) {
   return kAColTop - e;
 }
 
+template <int NT>
 __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     int lmax, int npairs, int nrings, int ncols, int nrg, int ncg, const double *__restrict__ z,
     const double *__restrict__ sth, const double2 *__restrict__ aab,
@@ -75,12 +76,14 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
   }
 
   // B operands: lane L holds B[k = 8 (L/32) + j][n = L % 32] of k-block kb,
-  // i.e. ring pair pw0 + 16 kb + 8 half + j, column col, for both parities.
-  const int col = cg * 32 + c32;
-  const bool colok = col < ncols;
-  uint4 bh[2][4], bl[2][4];
-  float fac;
-  {
+  // i.e. ring pair pw0 + 16 kb + 8 half + j, column col, for both parities;
+  // NT 32-column tiles per wave share the recurrence values.
+  uint4 bh[NT][2][4], bl[NT][2][4];
+  float fac[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) {
+    const int col = (cg * NT + ct) * 32 + c32;
+    const bool colok = col < ncols;
     float fn[4][8], fs[4][8];
     float mx = 0.0f;
 #pragma unroll
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       }
     mx = fmaxf(mx, __shfl_xor(mx, 32, 64));  // lanes L and L ^ 32 hold the same column
     const int sh = anal_shift(mx);
-    fac = colok ? ldexpf(1.0f, -(kLamExp + sh)) : 0.0f;
+    fac[ct] = colok ? ldexpf(1.0f, -(kLamExp + sh)) : 0.0f;
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
       uint32_t h[2][4], l[2][4];
@@ -112,10 +115,11 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       }
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
-        bh[p][kb] = uint4{h[p][0], h[p][1], h[p][2], h[p][3]};
-        bl[p][kb] = uint4{l[p][0], l[p][1], l[p][2], l[p][3]};
+        bh[ct][p][kb] = uint4{h[p][0], h[p][1], h[p][2], h[p][3]};
+        bl[ct][p][kb] = uint4{l[p][0], l[p][1], l[p][2], l[p][3]};
       }
     }
+    __builtin_amdgcn_sched_barrier(0);  // one tile's raw values live at a time
   }
 
   // LDS: per wave the transposed slab lam[hi|lo][parity][l' (32)][pair (64)]
@@ -198,40 +202,49 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     __builtin_amdgcn_wave_barrier();
 
     // MFMAs: A[l'][pair] of k-block kb: lane L reads row L % 32, pairs
-    // 16 kb + 8 (L / 32) .. + 7 (16 bytes)
-    f32x16 acc[2] = {f32x16{}, f32x16{}};
+    // 16 kb + 8 (L / 32) .. + 7 (16 bytes); one A read serves the NT tiles
+    f32x16 acc[NT][2];
+#pragma unroll
+    for (int ct = 0; ct < NT; ++ct) acc[ct][0] = acc[ct][1] = f32x16{};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         const f16x8 ah = *reinterpret_cast<const f16x8 *>(&lw[0][p][c32][16 * kb + 8 * half]);
         const f16x8 al = *reinterpret_cast<const f16x8 *>(&lw[1][p][c32][16 * kb + 8 * half]);
-        const f16x8 Bh = __builtin_bit_cast(f16x8, bh[p][kb]);
-        const f16x8 Bl = __builtin_bit_cast(f16x8, bl[p][kb]);
-        acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, Bh, acc[p], 0, 0, 0);
-        acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, Bl, acc[p], 0, 0, 0);
-        acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, Bh, acc[p], 0, 0, 0);
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          const f16x8 Bh = __builtin_bit_cast(f16x8, bh[ct][p][kb]);
+          const f16x8 Bl = __builtin_bit_cast(f16x8, bl[ct][p][kb]);
+          acc[ct][p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, Bh, acc[ct][p], 0, 0, 0);
+          acc[ct][p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, Bl, acc[ct][p], 0, 0, 0);
+          acc[ct][p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, Bh, acc[ct][p], 0, 0, 0);
+        }
       }
     }
 
     // fixed-order sum of the 8 waves' tiles: element e of lane L is row
-    // 8 (e/4) + 4 (L/32) + e%4 (l = l0 + 2 row + p), column L % 32
+    // 8 (e/4) + 4 (L/32) + e%4 (l = l0 + 2 row + p), column L % 32 of tile ct
     __syncthreads();  // every wave has read its lambda tile
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int ct = 0; ct < NT; ++ct)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) red[((wave * 2 + p) * 16 + e) * 64 + lane] = acc[p][e] * fac;
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          red[(((wave * NT + ct) * 2 + p) * 16 + e) * 64 + lane] = acc[ct][p][e] * fac[ct];
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int pe = wave * 4 + q;  // (parity, element) handled by this wave
-      const int p = pe >> 4, e = pe & 15;
+#pragma unroll 2
+    for (int q = 0; q < 4 * NT; ++q) {
+      const int pe = wave * 4 * NT + q;  // (tile, parity, element) handled by this wave
+      const int ct = pe >> 5, p = (pe >> 4) & 1, e = pe & 15;
       float sum = 0.0f;
 #pragma unroll
-      for (int w = 0; w < kAWaves; ++w) sum += red[((w * 2 + p) * 16 + e) * 64 + lane];
+      for (int w = 0; w < kAWaves; ++w) sum += red[(((w * NT + ct) * 2 + p) * 16 + e) * 64 + lane];
       const int row = 8 * (e >> 2) + 4 * half + (e & 3);
       const int l = l0 + 2 * row + p;
-      if (l <= lmax && colok)
+      const int col = (cg * NT + ct) * 32 + c32;
+      if (l <= lmax && col < ncols)
         partial[(int64_t(rg) * nalm + base + (l - m)) * ncols + col] =
             (m == 0 && (col & 1)) ? 0.0f : sum;
     }
@@ -247,13 +260,22 @@ int launch_legendre_anal_f16(const ga_plan *p, int S, const float *F, float *par
   const int nrg = (p->npairs + kAPairs - 1) / kAPairs;
   GA_REQUIRE(nrg == anal_partial_slots(p), GA_ERR_UNSUPPORTED, "partial slots %d != %d", nrg,
              anal_partial_slots(p));
-  const int ncg = (ncols + 31) / 32;
+  // two 32-column tiles per wave when there are more than 32 columns (the
+  // recurrence then serves 64 columns; the waves' tiles fill the reduction
+  // buffer exactly)
+  const int nt = ncols > 32 ? 2 : 1;
+  const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
   const int nm8 = (p->lmax + 1 + 7) / 8;
   const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
   GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
-  hipLaunchKernelGGL(legendre_anal_f16_kernel, dim3(unsigned(nblk)), dim3(64 * kAWaves), 0,
-                     stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth,
-                     p->d_aab, p->d_cm, F, partial, p->nalm);
+  if (nt == 2)
+    hipLaunchKernelGGL(legendre_anal_f16_kernel<2>, dim3(unsigned(nblk)), dim3(64 * kAWaves), 0,
+                       stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth,
+                       p->d_aab, p->d_cm, F, partial, p->nalm);
+  else
+    hipLaunchKernelGGL(legendre_anal_f16_kernel<1>, dim3(unsigned(nblk)), dim3(64 * kAWaves), 0,
+                       stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth,
+                       p->d_aab, p->d_cm, F, partial, p->nalm);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

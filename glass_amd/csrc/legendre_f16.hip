@@ -381,16 +381,16 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
                  colexp, F);
 }
 
-// Producer / consumer form for wide batches (more than 64 real columns).
-// The accumulators of 64 ring pairs x 64 columns fill a wave's register
-// budget at two waves per SIMD, so with 128 columns the single-wave form runs
-// the FP64 recurrence of every ring pair twice.  Here a workgroup of 8 waves
-// pairs wave w (producer: recurrence, splitting, MFMAs for columns 0..63 of
-// the 128-column group) with wave w + 4 (consumer: MFMAs for columns
-// 64..127) on the same 64 ring pairs; the producer passes each slab's A
-// operands through LDS (double-buffered, one workgroup barrier per slab), so
-// every recurrence value serves 128 columns and the consumer's MFMAs overlap
-// the producer's next slab on the VALU.
+// Producer / consumer form.  In the single-wave form every wave runs the
+// FP64 recurrence (VALU) and then its MFMAs; the two waves of a SIMD do the
+// same work in step, so the two phases add up instead of overlapping.  Here
+// a workgroup of 8 waves splits the roles on 4 groups of 64 ring pairs:
+// wave w (producer) runs the recurrence and the FP16 splitting and passes
+// each slab's A operands through LDS (double-buffered, one workgroup barrier
+// per slab); wave w + 4 (consumer) runs the MFMAs of that slab against its
+// 64 columns while the producer computes the next slab, and stores the
+// result.  The consumer keeps the B operands of the next two slabs in
+// registers, so their loads have two slab periods to arrive.
 __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
     int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
     const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
     const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
     const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F) {
   constexpr int NT = 2;
-  // XCD-aware decode as in the single-wave form (ncg counts 128-column groups)
+  // XCD-aware decode as in the single-wave form (ncg counts 64-column groups)
   const int b = blockIdx.x;
   const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
   const int m = 8 * (kq / G) + xcd;
@@ -406,26 +406,64 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
   const int rem = kq - (kq / G) * G;
   const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int pr = wave & 3;          // ring-pair group of the pair
+  const int pr = wave & 3;  // ring-pair group of the pair
   const bool producer = wave < 4;
   const int rp0 = rg * 256 + pr * 64;
   const bool active = rp0 < npairs;  // same for both waves of a pair
   const int rp = rp0 + lane;
   const int rpc = rp < npairs ? rp : npairs - 1;
-  const double x = z[rpc];
-  double p0, p1 = 0.0;
-  int k;
-  legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
-
   const int64_t base = hp_index(lmax, m, m);
-  const double2 *__restrict__ co = ab + base;
   const int64_t s0 = slab_off[m];
   const int nsl = int(nslab_of(lmax, m));
-  const int colbase = cg * 128 + (producer ? 0 : 64);
+
+  // A operands of a slab, [buffer][pair][hi p0t0 p0t1 p1t0 p1t1 | lo ...][lane],
+  // and whether the slab has any nonzero lambda
+  __shared__ __align__(16) uint4 abuf[2][4][8][64];
+  __shared__ int aflag[2][4];
+
+  if (producer) {
+    __shared__ double2 cbuf[4][2][kSlab];
+    const double x = z[rpc];
+    double p0, p1 = 0.0;
+    int k;
+    legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
+    const double2 *__restrict__ co = ab + base;
+    const int lastc = lmax - m;
+    double2 cpre = co[min(lane & (kSlab - 1), lastc)];
+    if (lane < kSlab) cbuf[pr][0][lane] = cpre;
+    __builtin_amdgcn_wave_barrier();
+    for (int s = 0; s < nsl; ++s) {
+      const int cur = s & 1;
+      if (active) {
+        cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
+        uint32_t ph[2][8], pl[2][8];
+        slab_values(cbuf[pr][cur], x, m + kSlab * s, lmax, p0, p1, k, ph, pl);
+        if (lane < kSlab) cbuf[pr][cur ^ 1][lane] = cpre;
+        __builtin_amdgcn_wave_barrier();
+        const bool need = wave_any(k == 0);
+        if (need) {
+          uint4 ah[2][2], al[2][2];
+          f16::slab_a(ph, pl, ah, al);
+#pragma unroll
+          for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+              abuf[cur][pr][2 * p + t][lane] = ah[p][t];
+              abuf[cur][pr][4 + 2 * p + t][lane] = al[p][t];
+            }
+        }
+        if (lane == 0) aflag[cur][pr] = need ? 1 : 0;
+      }
+      __syncthreads();  // slab s's A operands are in LDS
+    }
+    return;
+  }
+
+  // consumer
+  const int colbase = cg * 32 * NT;
   int ctile[NT];
 #pragma unroll
-  for (int ct = 0; ct < NT; ++ct) ctile[ct] = min(colbase / 32 + ct, nct - 1);
-
+  for (int ct = 0; ct < NT; ++ct) ctile[ct] = min(cg * NT + ct, nct - 1);
   f32x16 acc[2][2][NT];
 #pragma unroll
   for (int p = 0; p < 2; ++p)
@@ -433,55 +471,22 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
-
-  // A operands of a slab, [buffer][pair][hi p0t0 p0t1 p1t0 p1t1 | lo ...][lane],
-  // and whether the slab has any nonzero lambda
-  __shared__ __align__(16) uint4 abuf[2][4][8][64];
-  __shared__ int aflag[2][4];
-  __shared__ double2 cbuf[4][2][kSlab];  // the producers' coefficient buffers
-  const int lastc = lmax - m;
-  double2 cpre = double2{0.0, 0.0};
-  if (producer) {
-    cpre = co[min(lane & (kSlab - 1), lastc)];
-    if (lane < kSlab) cbuf[pr][0][lane] = cpre;
-    __builtin_amdgcn_wave_barrier();
-  }
-
-  for (int s = 0; s < nsl; ++s) {
-    const int l0 = m + kSlab * s;
-    const int cur = s & 1;
-    uint4 bh[2][NT], bl[2][NT];
+  // B of slabs s (bc) and s + 1 (bn); slab indices clamped at the end
+  uint4 bch[2][NT], bcl[2][NT], bnh[2][NT], bnl[2][NT];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+  for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < NT; ++ct) {
-        const uint4 *__restrict__ u = B + (((s0 + s) * 2 + p) * nct + ctile[ct]) * 128;
-        bh[p][ct] = u[lane];
-        bl[p][ct] = u[64 + lane];
-      }
-    if (producer && active) {
-      cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
-      uint32_t ph[2][8], pl[2][8];
-      slab_values(cbuf[pr][cur], x, l0, lmax, p0, p1, k, ph, pl);
-      if (lane < kSlab) cbuf[pr][cur ^ 1][lane] = cpre;
-      __builtin_amdgcn_wave_barrier();
-      const bool need = wave_any(k == 0);
-      if (need) {
-        uint4 ah[2][2], al[2][2];
-        f16::slab_a(ph, pl, ah, al);
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int t = 0; t < 2; ++t) {
-            abuf[cur][pr][2 * p + t][lane] = ah[p][t];
-            abuf[cur][pr][4 + 2 * p + t][lane] = al[p][t];
-          }
-      }
-      if (lane == 0) aflag[cur][pr] = need ? 1 : 0;
+    for (int ct = 0; ct < NT; ++ct) {
+      const uint4 *__restrict__ u0 = B + ((s0 * 2 + p) * nct + ctile[ct]) * 128;
+      const uint4 *__restrict__ u1 = B + (((s0 + min(1, nsl - 1)) * 2 + p) * nct + ctile[ct]) * 128;
+      bch[p][ct] = u0[lane];
+      bcl[p][ct] = u0[64 + lane];
+      bnh[p][ct] = u1[lane];
+      bnl[p][ct] = u1[64 + lane];
     }
-    __syncthreads();  // slab s's A operands are in LDS; buffer cur ^ 1 is free
-    // both waves of the pair read the A operands back (the producer too: its
-    // registers are then free across the barrier)
+  for (int s = 0; s < nsl; ++s) {
+    const int cur = s & 1;
+    __syncthreads();  // the producer has written slab s
     if (active && aflag[cur][pr] != 0) {
       uint4 ah[2][2], al[2][2];
 #pragma unroll
@@ -491,10 +496,21 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
           ah[p][t] = abuf[cur][pr][2 * p + t][lane];
           al[p][t] = abuf[cur][pr][4 + 2 * p + t][lane];
         }
-      f16::slab_mfma_a<NT>(ah, al, bh, bl, acc);
+      f16::slab_mfma_a<NT>(ah, al, bch, bcl, acc);
     }
+    // rotate the B pipeline and request slab s + 2
+    const int sn = min(s + 2, nsl - 1);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        bch[p][ct] = bnh[p][ct];
+        bcl[p][ct] = bnl[p][ct];
+        const uint4 *__restrict__ u = B + (((s0 + sn) * 2 + p) * nct + ctile[ct]) * 128;
+        bnh[p][ct] = u[lane];
+        bnl[p][ct] = u[64 + lane];
+      }
   }
-
   if (!active) return;
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
                  colexp, F);
@@ -537,11 +553,11 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
   const ga_tuning &tu = tuning();
   int nt = tu.mfma_nt;
-  if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : (ncols <= 64 ? 2 : 4);
+  if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : 2;
   if (nt == 4) {
-    // producer / consumer pairs over 128-column groups
+    // producer / consumer wave pairs over 64-column groups
     const int nrg = (p->npairs + 255) / 256;
-    const int ncg = (ncols + 127) / 128;
+    const int ncg = (ncols + 63) / 64;
     const int nm8 = (p->lmax + 1 + 7) / 8;
     const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
     GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");

@@ -157,37 +157,19 @@ __global__ __launch_bounds__(256, MINW) void spin_synth_f16_kernel(
     uint32_t ph[2][8], pl[2][8];
     const bool slow = (l0 + kSlab - 1 > lmax) || wave_any(k < 0);
     if (!slow) {
-      double2 ua[4], un[4];
-      double wa[4], wn[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        ua[j] = cb[j];
-        wa[j] = cc[j];
-      }
 #pragma unroll
       for (int g = 0; g < 8; ++g) {
-        if (g < 7) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            un[j] = cb[4 * (g + 1) + j];
-            wn[j] = cc[4 * (g + 1) + j];
-          }
-        }
         float v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const double pn = fma(fma(ua[j].x, x, ua[j].y), p1, -(wa[j] * p0));
+          const double2 u = cb[4 * g + j];
+          const double pn = fma(fma(u.x, x, u.y), p1, -(cc[4 * g + j] * p0));
           p0 = p1;
           p1 = pn;
           v[j] = float(pn);
         }
         split_pair(v[0], v[2], ph[0][g], pl[0][g]);
         split_pair(v[1], v[3], ph[1][g], pl[1][g]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          ua[j] = un[j];
-          wa[j] = wn[j];
-        }
         __builtin_amdgcn_sched_barrier(0);
       }
     } else {
@@ -320,9 +302,11 @@ int launch_spin_synth_f16(const ga_plan *p, int spin, int S, const float *E, con
   float *F = reinterpret_cast<float *>(base);
   void *bws = base + w.b_off;
   int32_t *colexp = reinterpret_cast<int32_t *>(base + w.e_off);
-  // one 32-column tile per wave: two tiles need more than the 256 VGPRs of
-  // two waves per SIMD (the accumulators and recurrences of both signs)
-  constexpr int nt = 1;
+  // two 32-column tiles per wave when there are more than 32 columns (each
+  // recurrence then serves 64 columns); the fast path reads its coefficients
+  // from LDS step by step, which keeps the two-tile form within the 256 VGPRs
+  // of two waves per SIMD
+  const int nt = ncols > 32 ? 2 : 1;
   const int nrg = (p->npairs + 4 * kSpinPairs - 1) / (4 * kSpinPairs);
   const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
   const int nm8 = (p->lmax + 1 + 7) / 8;
@@ -337,7 +321,8 @@ int launch_spin_synth_f16(const ga_plan *p, int spin, int S, const float *E, con
                      p->d_sth, p->d_spin[spin], p->d_spin_seed[spin], p->d_ring_pnum,            \
                      p->d_ring_pden, p->d_slab_off, static_cast<const uint4 *>(bws), colexp, F,  \
                      pass)
-    GA_SPIN_LAUNCH(1, 2);
+    if (nt == 2) GA_SPIN_LAUNCH(2, 2);
+    else GA_SPIN_LAUNCH(1, 2);
 #undef GA_SPIN_LAUNCH
     GA_LAUNCH_CHECK();
   }

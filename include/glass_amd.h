@@ -53,9 +53,8 @@ typedef struct ga_plan ga_plan;
 /* Process-wide tuning knobs of the batched MFMA Legendre stage (set by the
  * autotuner; no counterpart in the reference, which has no device code). */
 enum ga_tune_key {
-  GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2, 4 (split FP16 only: two
-                             waves share each recurrence over 128 columns) or 0 =
-                             automatic */
+  GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2, 0 = automatic, or 4 =
+                             producer / consumer wave pairs (split FP16 only) */
   GA_TUNE_MFMA_CHUNK = 2, /* FP32 path: quads per accumulation chunk, 0 = none */
   GA_TUNE_MFMA_PATH = 3,  /* 0 = automatic (split FP16), 1 = FP32 MFMA, 2 = split FP16 */
   GA_TUNE_RING_PATH = 4,  /* FP32 ring synthesis: 0 = automatic (= 2, default),

@@ -184,7 +184,7 @@ def test_spin_mfma_matches_valu_and_fp64(spin, nside, lmax, S):
 
 
 @pytest.mark.parametrize("nside,lmax,S,niter", [(8, 20, 3, 0), (64, 150, 7, 0), (256, 600, 5, 1),
-                                                (512, 1023, 17, 0)])
+                                                (512, 1023, 17, 0), (128, 300, 40, 0)])
 def test_analysis_mfma_matches_valu_and_fp64(nside, lmax, S, niter):
     """FP32 batched analysis: the split-FP16 MFMA contraction against the VALU
     one and against the FP64 path (max |error| / RMS of the alm < 1e-5)."""
@@ -216,8 +216,9 @@ def test_analysis_mfma_matches_valu_and_fp64(nside, lmax, S, niter):
 @pytest.mark.parametrize("nside,lmax,S", [(32, 63, 40), (128, 255, 64), (64, 150, 100)])
 def test_wave_pair_legendre_matches_single_wave(nside, lmax, S):
     """The producer / consumer wave-pair form of the split-FP16 Legendre stage
-    (tiles = 4) issues the same MFMAs in the same order as the single-wave
-    form with two tiles, so the maps agree bit for bit."""
+    (tiles = 4: one wave runs the recurrence, its partner the MFMAs) issues
+    the same MFMAs in the same order as the single-wave form with two tiles,
+    so the maps agree bit for bit."""
     from glass_amd import autotune
 
     print(f"wave pairs nside={nside} lmax={lmax} S={S}", flush=True)
