@@ -113,7 +113,11 @@ struct ga_plan {
   // rings by FFT class (power-of-two r / Bluestein), with the class maxima
   int32_t *d_rings_pow2 = nullptr, *d_rings_blu = nullptr;
   int32_t n_pow2 = 0, n_blu = 0, max_r_pow2 = 0, max_r_blu = 0, max_L_blu = 0;
-  double2 *d_tw64 = nullptr;  // exp(2 pi i j / max_L), j < max_L
+  // twiddles exp(2 pi i j / tw_len), j < tw_len; tw_len = max(max_L, 4 max_r_pow2)
+  // (a power of two), so the radix-4 split of a power-of-two ring reads its
+  // twiddles from the table too
+  int32_t tw_len = 0;
+  double2 *d_tw64 = nullptr;
   float2 *d_tw32 = nullptr;
   size_t table_bytes = 0;
 };

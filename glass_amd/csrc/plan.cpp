@@ -316,11 +316,12 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
     std::vector<int64_t> off(size_t(max_r + 1), -1);
     std::vector<char> used(size_t(max_r + 1), 0);
     for (int r = 0; r < p->nrings; ++r) used[size_t(rn[size_t(r)] / 4)] = 1;
-    int64_t total = 0, maxL = 1;
+    int64_t total = 0, maxL = 1, maxn_pow2 = 1;
     for (int32_t r = 1; r <= max_r; ++r) {
       if (!used[size_t(r)]) continue;
       if (is_pow2(r)) {
         maxL = std::max<int64_t>(maxL, r);
+        maxn_pow2 = std::max<int64_t>(maxn_pow2, 4 * int64_t(r));
         continue;
       }
       int64_t L = blu_len(r);
@@ -354,10 +355,12 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
     }
     std::vector<float2> blu32(blu.size());
     for (size_t i = 0; i < blu.size(); ++i) blu32[i] = float2{float(blu[i].x), float(blu[i].y)};
-    std::vector<double2> tw(static_cast<size_t>(maxL));
-    std::vector<float2> tw32(static_cast<size_t>(maxL));
-    for (int64_t j = 0; j < maxL; ++j) {
-      double ang = 2.0 * M_PI * double(j) / double(maxL);
+    const int64_t twlen = std::max(maxL, maxn_pow2);
+    p->tw_len = int32_t(twlen);
+    std::vector<double2> tw(static_cast<size_t>(twlen));
+    std::vector<float2> tw32(static_cast<size_t>(twlen));
+    for (int64_t j = 0; j < twlen; ++j) {
+      double ang = 2.0 * M_PI * double(j) / double(twlen);
       tw[size_t(j)] = double2{std::cos(ang), std::sin(ang)};
       tw32[size_t(j)] = float2{float(tw[size_t(j)].x), float(tw[size_t(j)].y)};
     }

@@ -247,10 +247,37 @@ __device__ void sub_idft(cplx<T> (&H)[MAXC][4][P], int slot, int r, cplx<T> *X, 
   }
 }
 
+// e^x - 1 in FP32 by Kahan's formula (N. J. Higham, Accuracy and Stability
+// of Numerical Algorithms, 2nd ed., sec. 1.14.1): with u = e^a rounded,
+// (u - 1) a / ln u is accurate to a few ulp for every a, small |a| included,
+// because the error of u enters only through (e^t - 1) / t at a point t
+// within an ulp of a.  Here a = y ln 2 with y = x log2(e) rounded to FP32,
+// u = 2^y, a / ln u = y / log2 u (hardware exp2 / log2 / reciprocal), and the
+// rounding of y, d = x - y ln 2 (from the exact FMA residual of x log2(e) and
+// the low part of log2(e)), is added back as e^(a + d) - 1 = (e^a - 1) + u d.
+// Relative error a few 1e-7 over the whole range.
+__device__ __forceinline__ float expm1_kahan(float x) {
+  constexpr float kLog2eHi = 1.44269502162933350f;  // log2(e) rounded to FP32
+  constexpr float kLog2eLo = 1.92596299112661746e-08f;  // log2(e) - kLog2eHi
+  constexpr float kLn2 = 0.693147180559945309f;
+  const float y = x * kLog2eHi;
+  const float ylo = fmaf(x, kLog2eLo, fmaf(x, kLog2eHi, -y));
+  const float u = __builtin_amdgcn_exp2f(y);
+  const float um1 = u - 1.0f;
+  const float e = um1 * (y * __builtin_amdgcn_rcpf(__builtin_amdgcn_logf(u)));
+  float r = fmaf(u, ylo * kLn2, e);
+  if (u == 1.0f) r = x;                 // |x| below half an ulp of 1
+  if (um1 == -1.0f) r = -1.0f;          // e^x underflows
+  if (x > 88.0f) r = __builtin_inff();  // e^x overflows
+  return r;
+}
+__device__ __forceinline__ float expm1_t(float x) { return expm1_kahan(x); }
+__device__ __forceinline__ double expm1_t(double x) { return expm1(x); }
+
 template <typename T>
 __device__ __forceinline__ T apply_epi(T x, int kind, T lam, T par) {
   if (kind == GA_T_LOGNORMAL) {
-    T y = expm1(x - par * T(0.5));
+    T y = expm1_t(x - par * T(0.5));
     return lam == T(1) ? y : lam * y;
   }
   if (kind == GA_T_SQUARED_NORMAL) {
@@ -638,25 +665,33 @@ __device__ __forceinline__ cplx<float> twid(const float2 *__restrict__ tw, int i
 // One pass fusing K radix-2 stages with half-spans h0, 2 h0, .., 2^(K-1) h0
 // (DIT: stages in increasing span; DIF: decreasing) on A arrays of length N.
 // Group = elements base + q h0 (q < 2^K), base = blk 2^K h0 + b, b < h0.
+// A group either lies inside one 32-element block or starts on a multiple of
+// 32 with h0 dividing 32 or a multiple of it, so its padded slots are
+//   pad(base + q h0) = pad(base) + q h0 + (q h0 >> 5),
+// offsets that are the same for every thread of the pass.  tw holds
+// e^{2 pi i j / 2^ltw}.
 template <int K, int NT, bool DIT, int SIGN>
 __device__ __forceinline__ void fused_pass(cplx<float> *X, int ldx, int A, int N, int h0,
-                                           const float2 *__restrict__ tw, int maxL) {
+                                           const float2 *__restrict__ tw, int ltw) {
   constexpr int R = 1 << K;
   const int lh0 = ilog2(h0);
   const int lg = ilog2(N) - K;  // groups per array = N / R
   const int total = A << lg;
+  int off[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) off[q] = (q << lh0) + ((q << lh0) >> 5);
   for (int gg = threadIdx.x; gg < total; gg += NT) {
     const int a = gg >> lg, g = gg & ((1 << lg) - 1);
     const int b = g & (h0 - 1), blk = g >> lh0;
-    cplx<float> *Y = X + a * ldx;
     const int base = (blk << (K + lh0)) + b;
+    cplx<float> *Y = X + a * ldx + pad(base);
     cplx<float> v[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = Y[pad(base + (q << lh0))];
+    for (int q = 0; q < R; ++q) v[q] = Y[off[q]];
     // ws[s] = e^{SIGN 2 pi i b / (2^(s+1) h0)}: one table value per stage
     cplx<float> ws[K];
 #pragma unroll
-    for (int s = 0; s < K; ++s) ws[s] = twid(tw, (b * (maxL >> (s + 1 + lh0))), SIGN);
+    for (int s = 0; s < K; ++s) ws[s] = twid(tw, b << (ltw - s - 1 - lh0), SIGN);
 #pragma unroll
     for (int ss = 0; ss < K; ++ss) {
       const int s = DIT ? ss : K - 1 - ss;
@@ -679,7 +714,7 @@ __device__ __forceinline__ void fused_pass(cplx<float> *X, int ldx, int A, int N
       }
     }
 #pragma unroll
-    for (int q = 0; q < R; ++q) Y[pad(base + (q << lh0))] = v[q];
+    for (int q = 0; q < R; ++q) Y[off[q]] = v[q];
   }
   __syncthreads();
 }
@@ -687,17 +722,17 @@ __device__ __forceinline__ void fused_pass(cplx<float> *X, int ldx, int A, int N
 // full power-of-two transform of A arrays: DIT (bit-reversed in, natural out)
 // or DIF (natural in, bit-reversed out)
 template <int NT, bool DIT, int SIGN>
-__device__ void fused_fft(cplx<float> *X, int ldx, int A, int N, const float2 *__restrict__ tw,
-                          int maxL) {
-  const int lN = ilog2(N);
+__device__ __forceinline__ void fused_fft(cplx<float> *X, int ldx, int A, int N,
+                                          const float2 *__restrict__ tw, int twlen) {
+  const int lN = ilog2(N), ltw = ilog2(twlen);
   if (DIT) {
     for (int l = 0; l < lN;) {
       const int K = lN - l >= 4 ? 4 : lN - l;
       switch (K) {
-        case 4: fused_pass<4, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
-        case 3: fused_pass<3, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
-        case 2: fused_pass<2, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
-        default: fused_pass<1, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, maxL); break;
+        case 4: fused_pass<4, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, ltw); break;
+        case 3: fused_pass<3, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, ltw); break;
+        case 2: fused_pass<2, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, ltw); break;
+        default: fused_pass<1, NT, true, SIGN>(X, ldx, A, N, 1 << l, tw, ltw); break;
       }
       l += K;
     }
@@ -706,10 +741,10 @@ __device__ void fused_fft(cplx<float> *X, int ldx, int A, int N, const float2 *_
       const int K = l >= 4 ? 4 : l;
       const int h0 = 1 << (l - K);
       switch (K) {
-        case 4: fused_pass<4, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
-        case 3: fused_pass<3, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
-        case 2: fused_pass<2, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
-        default: fused_pass<1, NT, false, SIGN>(X, ldx, A, N, h0, tw, maxL); break;
+        case 4: fused_pass<4, NT, false, SIGN>(X, ldx, A, N, h0, tw, ltw); break;
+        case 3: fused_pass<3, NT, false, SIGN>(X, ldx, A, N, h0, tw, ltw); break;
+        case 2: fused_pass<2, NT, false, SIGN>(X, ldx, A, N, h0, tw, ltw); break;
+        default: fused_pass<1, NT, false, SIGN>(X, ldx, A, N, h0, tw, ltw); break;
       }
       l -= K;
     }
@@ -723,21 +758,32 @@ __device__ void fused_fft(cplx<float> *X, int ldx, int A, int N, const float2 *_
 // q + Q, ... of m, each holding the direct term m = j n + k and the conjugate
 // term m = (j + 1) n - k.  Small rings (n <= NT / 2) get Q = NT / n threads per
 // bin, whose partial sums meet in LDS and are added in q order; a bin's direct
-// terms are summed before its conjugate terms in both cases.  The loads of four
-// chunks (or, when every bin has one chunk, of four bins) are issued together
-// at clamped addresses, so a thread never waits on one load at a time.
-template <int NT>
-__device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, int G, bool two,
-                                          int n, int r, int lmax, int lbits, int ldx,
-                                          cplx<float> *X) {
+// terms are summed before its conjugate terms in both cases.  The loads of
+// kBatch chunks (or, when every bin has one chunk, of kBatch bins) are issued
+// together at clamped addresses, so a thread keeps many loads in flight.
+// MODE: 0 = one shell, 1 = two shells (two 8-byte loads per m), 2 = two
+// shells from one 16-byte load (the pair 16-byte aligned).
+constexpr int kBatch = 8;
+
+template <int MODE>
+__device__ __forceinline__ void load_pair(const cplx<float> *__restrict__ f, cplx<float> &a,
+                                          cplx<float> &b) {
+  if (MODE == 2) {
+    const float4 v = *reinterpret_cast<const float4 *>(f);
+    a = cplx<float>{v.x, v.y};
+    b = cplx<float>{v.z, v.w};
+  } else {
+    a = f[0];
+    b = MODE == 1 ? f[1] : cplx<float>{0.0f, 0.0f};
+  }
+}
+
+template <int NT, int MODE>
+__device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, int G, int n, int r,
+                                          int lmax, int lbits, int ldx, cplx<float> *X) {
   const int t = threadIdx.x;
   const int Q = n <= NT / 2 ? NT / n : 1;
   const int jmax = lmax / n;  // chunks 0..jmax can hold terms
-  auto ld = [&](int m, cplx<float> &a, cplx<float> &b) {
-    const cplx<float> *f = Fu + int64_t(m) * G;
-    a = f[0];
-    b = two ? f[1] : cplx<float>{0.0f, 0.0f};
-  };
   auto slot = [&](int k) {
     const int k2 = (k >= r) + (k >= 2 * r) + (k >= 3 * r);
     const int k1 = k - k2 * r;
@@ -749,22 +795,30 @@ __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, in
   };
   if (jmax == 0) {
     // n > lmax: a bin has at most one direct and one conjugate term
-#pragma unroll 4
-    for (int k = t; k < n; k += NT) {
-      cplx<float> d0, d1, c0, c1;
-      const int mc = n - k;
-      ld(min(k, lmax), d0, d1);
-      ld(min(mc, lmax), c0, c1);
-      cplx<float> g0 = {0.0f, 0.0f}, g1 = {0.0f, 0.0f};
-      if (k <= lmax) {
-        g0 = d0;
-        g1 = d1;
+    for (int k0 = t; k0 < n; k0 += kBatch * NT) {
+      cplx<float> fd[kBatch][2], fc[kBatch][2];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int k = min(k0 + u * NT, n - 1);
+        load_pair<MODE>(Fu + int64_t(min(k, lmax)) * G, fd[u][0], fd[u][1]);
+        load_pair<MODE>(Fu + int64_t(min(n - k, lmax)) * G, fc[u][0], fc[u][1]);
       }
-      if (mc <= lmax) {
-        g0 = cadd(g0, cconj(c0));
-        g1 = cadd(g1, cconj(c1));
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int k = k0 + u * NT;
+        if (k < n) {
+          cplx<float> g0 = {0.0f, 0.0f}, g1 = {0.0f, 0.0f};
+          if (k <= lmax) {
+            g0 = fd[u][0];
+            g1 = fd[u][1];
+          }
+          if (n - k <= lmax) {
+            g0 = cadd(g0, cconj(fc[u][0]));
+            g1 = cadd(g1, cconj(fc[u][1]));
+          }
+          X[slot(k)] = pack(g0, g1);
+        }
       }
-      X[slot(k)] = pack(g0, g1);
     }
     return;
   }
@@ -773,16 +827,18 @@ __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, in
   for (int it = t; it < nitems; it += NT) {
     const int q = it / n, k = it - q * n;
     cplx<float> d0 = {0.0f, 0.0f}, d1 = d0, c0 = d0, c1 = d0;
-    for (int j0 = q; j0 <= jmax; j0 += 4 * Q) {
-      cplx<float> fd[4][2], fc[4][2];
+    // chunks j0, j0 + Q, ..: full batches of kBatch, then pairs
+    auto batch = [&](auto nb, int j0) {
+      constexpr int B = decltype(nb)::value;
+      cplx<float> fd[B][2], fc[B][2];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < B; ++u) {
         const int j = j0 + u * Q;
-        ld(min(j * n + k, lmax), fd[u][0], fd[u][1]);
-        ld(min((j + 1) * n - k, lmax), fc[u][0], fc[u][1]);
+        load_pair<MODE>(Fu + int64_t(min(j * n + k, lmax)) * G, fd[u][0], fd[u][1]);
+        load_pair<MODE>(Fu + int64_t(min((j + 1) * n - k, lmax)) * G, fc[u][0], fc[u][1]);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < B; ++u) {
         const int j = j0 + u * Q;
         if (j * n + k <= lmax) {
           d0 = cadd(d0, fd[u][0]);
@@ -793,7 +849,11 @@ __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, in
           c1 = cadd(c1, cconj(fc[u][1]));
         }
       }
-    }
+    };
+    int j0 = q;
+    for (; j0 + (kBatch - 1) * Q <= jmax; j0 += kBatch * Q)
+      batch(std::integral_constant<int, kBatch>{}, j0);
+    for (; j0 <= jmax; j0 += 2 * Q) batch(std::integral_constant<int, 2>{}, j0);
     if (Q == 1) {
       X[slot(k)] = pack(cadd(d0, c0), cadd(d1, c1));
     } else {
@@ -826,7 +886,7 @@ template <int NT, int MAXC, bool BLU>
 __global__ __launch_bounds__(NT) void ring_synth2_kernel(
     int lmax, const int32_t *__restrict__ ring_list, int nlist, const int64_t *__restrict__ ring_start,
     const int32_t *__restrict__ ring_n, const int64_t *__restrict__ blu_off,
-    const float2 *__restrict__ blu, const float2 *__restrict__ tw, int maxL, int ldx,
+    const float2 *__restrict__ blu, const float2 *__restrict__ tw, int twlen, int ldx,
     const cplx<float> *__restrict__ F, int S, int G, int nunit, float *__restrict__ maps,
     int64_t npix, const ga_epilogue *__restrict__ epi) {
   extern __shared__ char smem_raw[];
@@ -851,7 +911,13 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
   // (m <= lmax).  Bin k = k1 + r k2 is kept at X[k2 ldx + pad(pos(k1))], the
   // slot the radix-4 split below reads and rewrites (pos = bit-reversed k1 for
   // the DIT passes of a power-of-two r, k1 for Bluestein).
-  fold_bins<NT>(Fu, G, two, n, r, lmax, BLU ? 0 : lbits, ldx, X);
+  const int lb = BLU ? 0 : lbits;
+  if (!two)
+    fold_bins<NT, 0>(Fu, G, n, r, lmax, lb, ldx, X);
+  else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
+    fold_bins<NT, 2>(Fu, G, n, r, lmax, lb, ldx, X);
+  else
+    fold_bins<NT, 1>(Fu, G, n, r, lmax, lb, ldx, X);
   __syncthreads();
 
   // ---- radix-4 split (thread owns k1 = t + i NT), sub-transform inputs in place
@@ -863,7 +929,9 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
       cplx<float> h[4];
 #pragma unroll
       for (int k2 = 0; k2 < 4; ++k2) h[k2] = X[k2 * ldx + pad(pos)];
-      const cplx<float> w1 = expi_q(k1, n, 0.0f);
+      // e^{2 pi i k1 / n}: from the table when n (a power of two) divides its length
+      const cplx<float> w1 =
+          BLU ? expi_q(k1, n, 0.0f) : twid(tw, k1 << (ilog2(twlen) - ilog2(n)), 1);
       const cplx<float> w2 = cmul(w1, w1);
       const cplx<float> w3 = cmul(w2, w1);
       const cplx<float> s02 = cadd(h[0], h[2]), d02 = csub(h[0], h[2]);
@@ -889,7 +957,7 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
 
   // ---- the four length-r inverse DFTs (Bluestein: forward, chirp spectrum, inverse)
   if (BLU) {
-    fused_fft<NT, false, -1>(X, ldx, 4, L, tw, maxL);  // natural -> bit-reversed
+    fused_fft<NT, false, -1>(X, ldx, 4, L, tw, twlen);  // natural -> bit-reversed
     for (int e = t; e < L * 4; e += NT) {
       const int a = e >> lbits, j = e & (L - 1);
       const float2 b = bspec[j];  // stored bit-reversed
@@ -897,7 +965,7 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
     }
     __syncthreads();
   }
-  fused_fft<NT, true, +1>(X, ldx, 4, L, tw, maxL);  // bit-reversed -> natural
+  fused_fft<NT, true, +1>(X, ldx, 4, L, tw, twlen);  // bit-reversed -> natural
 
   // ---- epilogue: pixels 4 j1 .. 4 j1 + 3 of each shell, one 16-byte store
   const int64_t st = ring_start[ring];
@@ -950,7 +1018,7 @@ int launch_synth2(const ga_plan *p, const int32_t *list, int nlist, int N, int S
                                    int(lds)));
   hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, list, nlist,
                      p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32, p->d_tw32,
-                     p->max_L, ldx, reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps,
+                     p->tw_len, ldx, reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps,
                      p->npix, epi);
   GA_LAUNCH_CHECK();
   return GA_OK;
@@ -1018,7 +1086,7 @@ int launch_ring(const ga_plan *p, int S, int G, const void *in, void *out, const
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
     hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,           \
                        p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,          \
-                       p->d_blu_off, blu, tw, p->max_L, (const T *)in, S, npp, p->npix,       \
+                       p->d_blu_off, blu, tw, p->tw_len, (const T *)in, S, npp, p->npix,       \
                        T(4.0 * M_PI / double(p->npix)), (cplx<T> *)out);                       \
   } while (0)
     if (max_r <= 256) GA_ANAL_LAUNCH(256, 1);
@@ -1052,7 +1120,7 @@ int launch_ring(const ga_plan *p, int S, int G, const void *in, void *out, const
     GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn,                                       \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));  \
     hipLaunchKernelGGL(kfn, dim3(nblk), dim3(NT_), lds, stream, p->lmax, p->nrings,           \
-                       p->d_ring_start, p->d_ring_n, p->d_blu_off, blu, tw, p->max_L, ldx,    \
+                       p->d_ring_start, p->d_ring_n, p->d_blu_off, blu, tw, p->tw_len, ldx,    \
                        (const cplx<T> *)in, S, G, nunit, (T *)out, p->npix, epi);             \
   } while (0)
   if constexpr (sizeof(T) == 4) {
