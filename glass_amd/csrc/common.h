@@ -48,7 +48,7 @@ struct ga_tuning {
   int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto; 4 = wave pairs)
   int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
   int mfma_path = 0;    // Legendre stage: 0 auto (= split FP16), 1 FP32 MFMA, 2 split FP16
-  int ring_path = 0;    // FP32 ring synthesis: 0 auto (= fused passes), 1 radix-4 stages, 2 fused passes
+  int ring_path = 0;    // FP32 ring synthesis: 0 auto (fused passes, register fold on wide rings), 1 radix-4 stages, 2 fused passes only
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
 };
@@ -113,6 +113,16 @@ struct ga_plan {
   // rings by FFT class (power-of-two r / Bluestein), with the class maxima
   int32_t *d_rings_pow2 = nullptr, *d_rings_blu = nullptr;
   int32_t n_pow2 = 0, n_blu = 0, max_r_pow2 = 0, max_r_blu = 0, max_L_blu = 0;
+  // the power-of-two rings split once more: "wide" rings (r = r_wide, the
+  // largest power-of-two r, with 4 r > lmax, so every bin has at most one
+  // direct and one conjugate term; 256 <= r_wide <= 4096) and the rest
+  int32_t *d_rings_wide = nullptr, *d_rings_narrow = nullptr;
+  int32_t n_wide = 0, n_narrow = 0, r_wide = 0, max_r_narrow = 0;
+  // the Bluestein rings split the same way: wide (4 r > lmax, r <= 2048,
+  // L <= 4096) and the rest, with the class maxima
+  int32_t *d_rings_bwide = nullptr, *d_rings_bnarrow = nullptr;
+  int32_t n_bwide = 0, n_bnarrow = 0, max_r_bwide = 0, max_L_bwide = 0;
+  int32_t max_r_bnarrow = 0, max_L_bnarrow = 0;
   // twiddles exp(2 pi i j / tw_len), j < tw_len; tw_len = max(max_L, 4 max_r_pow2)
   // (a power of two), so the radix-4 split of a power-of-two ring reads its
   // twiddles from the table too

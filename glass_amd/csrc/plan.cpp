@@ -379,6 +379,45 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
     }
     p->n_pow2 = int32_t(lp.size());
     p->n_blu = int32_t(lb.size());
+    std::vector<int32_t> lw, ln;
+    const int32_t rw = p->max_r_pow2;
+    const bool wide_ok = rw >= 256 && rw <= 4096 && 4 * int64_t(rw) > lmax;
+    for (int32_t ring : lp) {
+      const int32_t r = rn[size_t(ring)] / 4;
+      if (wide_ok && r == rw) {
+        lw.push_back(ring);
+      } else {
+        ln.push_back(ring);
+        p->max_r_narrow = std::max(p->max_r_narrow, r);
+      }
+    }
+    std::vector<int32_t> lbw, lbn;
+    for (int32_t ring : lb) {
+      const int32_t r = rn[size_t(ring)] / 4;
+      const int32_t L = int32_t(blu_len(r));
+      if (4 * int64_t(r) > lmax && r <= 2048 && L <= 4096) {
+        lbw.push_back(ring);
+        p->max_r_bwide = std::max(p->max_r_bwide, r);
+        p->max_L_bwide = std::max(p->max_L_bwide, L);
+      } else {
+        lbn.push_back(ring);
+        p->max_r_bnarrow = std::max(p->max_r_bnarrow, r);
+        p->max_L_bnarrow = std::max(p->max_L_bnarrow, L);
+      }
+    }
+    p->n_bwide = int32_t(lbw.size());
+    p->n_bnarrow = int32_t(lbn.size());
+    if (!lbw.empty() && (rc = upload(&p->d_rings_bwide, lbw.data(), lbw.size() * 4, &p->table_bytes)))
+      return fail(rc);
+    if (!lbn.empty() && (rc = upload(&p->d_rings_bnarrow, lbn.data(), lbn.size() * 4, &p->table_bytes)))
+      return fail(rc);
+    p->n_wide = int32_t(lw.size());
+    p->n_narrow = int32_t(ln.size());
+    p->r_wide = wide_ok ? rw : 0;
+    if (!lw.empty() && (rc = upload(&p->d_rings_wide, lw.data(), lw.size() * 4, &p->table_bytes)))
+      return fail(rc);
+    if (!ln.empty() && (rc = upload(&p->d_rings_narrow, ln.data(), ln.size() * 4, &p->table_bytes)))
+      return fail(rc);
     if (!lp.empty() && (rc = upload(&p->d_rings_pow2, lp.data(), lp.size() * 4, &p->table_bytes)))
       return fail(rc);
     if (!lb.empty() && (rc = upload(&p->d_rings_blu, lb.data(), lb.size() * 4, &p->table_bytes)))
@@ -398,7 +437,8 @@ extern "C" int ga_plan_destroy(ga_plan *p) {
                   p->d_ab, p->d_aab, p->d_cm, p->d_slab_off, p->d_spin[0], p->d_spin[1], p->d_spin[2],
                   p->d_spin_seed[0], p->d_spin_seed[1], p->d_spin_seed[2], p->d_blu_off,
                   p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32, p->d_rings_pow2,
-                  p->d_rings_blu};
+                  p->d_rings_blu, p->d_rings_wide, p->d_rings_narrow, p->d_rings_bwide,
+                  p->d_rings_bnarrow};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete p;

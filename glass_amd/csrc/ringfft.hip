@@ -1024,22 +1024,391 @@ int launch_synth2(const ga_plan *p, const int32_t *list, int nlist, int N, int S
   return GA_OK;
 }
 
-// both ring classes with the fused-pass kernel; GA_ERR_UNSUPPORTED (before any
+// ---------------------------------------------------------------------------
+// FP32 synthesis of the wide power-of-two rings (n = 4 r > lmax, r = NT C):
+// every bin has at most one direct and one conjugate term, so a thread loads
+// the coefficients of its own bins k1 + r k2 (k1 = t + i NT, k2 = 0..3) and
+// runs the radix-4 split in registers: no LDS round trip for the fold.  The
+// four length-r transforms then run two at a time on two LDS arrays (half
+// the LDS of the four-array form, so twice the workgroups per CU): outputs 2
+// and 3 of the split wait in registers while arrays 0 and 1 are transformed,
+// and the results of 0 and 1 wait in registers while 2 and 3 are.  Same
+// arithmetic, term order and passes as ring_synth2_kernel.
+// Bin k of a ring with n > lmax holds at most the direct term F_k (k <= lmax)
+// and the conjugate term conj(F_{n-k}) (n - k <= lmax).  wide_bin_first loads
+// the first present one (conjugated if it is the conjugate term) into g;
+// wide_bin_second adds the conjugate term where both are present (bins
+// n - lmax <= k <= lmax, few or none), so only one load per bin is in flight.
+template <int MODE>
+__device__ __forceinline__ void wide_bin_first(const cplx<float> *__restrict__ Fu, int G, int n,
+                                               int lmax, int k, bool valid, cplx<float> (&g)[2]) {
+  const bool direct = k <= lmax;
+  g[0] = g[1] = cplx<float>{0.0f, 0.0f};
+  if (valid && (direct || n - k <= lmax))
+    load_pair<MODE>(Fu + int64_t(direct ? k : n - k) * G, g[0], g[1]);
+  if (!direct) {
+    g[0] = cconj(g[0]);
+    g[1] = cconj(g[1]);
+  }
+}
+template <int MODE>
+__device__ __forceinline__ void wide_bin_second(const cplx<float> *__restrict__ Fu, int G, int n,
+                                                int lmax, int k, bool valid, cplx<float> (&g)[2]) {
+  if (valid && k <= lmax && n - k <= lmax) {
+    cplx<float> c0, c1;
+    load_pair<MODE>(Fu + int64_t(n - k) * G, c0, c1);
+    g[0] = cadd(g[0], cconj(c0));
+    g[1] = cadd(g[1], cconj(c1));
+  }
+}
+
+template <int NT, int C, int MODE>
+__device__ __forceinline__ void fold_split_wide(const cplx<float> *__restrict__ Fu, int G, int r,
+                                                int lmax, const float2 *__restrict__ tw,
+                                                int ltw, int ldx, cplx<float> *X,
+                                                cplx<float> (&park)[C][2]) {
+  const int t = threadIdx.x;
+  const int n = 4 * r, ln = ilog2(n), lbits = ilog2(r);
+  constexpr int B = C >= 2 ? 2 : 1;  // k1 values whose loads are issued together
+#pragma unroll
+  for (int i0 = 0; i0 < C; i0 += B) {
+    cplx<float> g[B][4][2];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2)
+        wide_bin_first<MODE>(Fu, G, n, lmax, t + (i0 + u) * NT + k2 * r, true, g[u][k2]);
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2)
+        wide_bin_second<MODE>(Fu, G, n, lmax, t + (i0 + u) * NT + k2 * r, true, g[u][k2]);
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int k1 = t + (i0 + u) * NT;
+      cplx<float> h[4];
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2)
+        h[k2] = cplx<float>{g[u][k2][0].re - g[u][k2][1].im, g[u][k2][0].im + g[u][k2][1].re};
+      const cplx<float> w1 = twid(tw, k1 << (ltw - ln), 1);
+      const cplx<float> w2 = cmul(w1, w1);
+      const cplx<float> w3 = cmul(w2, w1);
+      const cplx<float> s02 = cadd(h[0], h[2]), d02 = csub(h[0], h[2]);
+      const cplx<float> s13 = cadd(h[1], h[3]), d13 = cmuli(csub(h[1], h[3]));
+      const int pos = int(__brev(unsigned(k1)) >> (32 - lbits));
+      X[pad(pos)] = cadd(s02, s13);
+      X[ldx + pad(pos)] = cmul(cadd(d02, d13), w1);
+      park[i0 + u][0] = cmul(csub(s02, s13), w2);
+      park[i0 + u][1] = cmul(csub(d02, d13), w3);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // one batch of loads in flight
+  }
+}
+
+template <int NT, int C>
+__global__ __launch_bounds__(NT, 4) void ring_synth_wide_kernel(
+    int lmax, const int32_t *__restrict__ ring_list, int nlist,
+    const int64_t *__restrict__ ring_start, int r, const float2 *__restrict__ tw, int twlen,
+    int ldx, const cplx<float> *__restrict__ F, int S, int G, int nunit,
+    float *__restrict__ maps, int64_t npix, const ga_epilogue *__restrict__ epi) {
+  extern __shared__ char smem_raw[];
+  cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
+  int li, unit;
+  if (!decode_ring_block(nlist, nunit, li, unit)) return;
+  const int ring = ring_list[li];
+  const int t = threadIdx.x;
+  const int sbase = 2 * unit;
+  const int ngrp = (S + G - 1) / G;
+  const cplx<float> *Fu = F + (int64_t(ring) * ngrp + sbase / G) * (lmax + 1) * G + sbase % G;
+  const bool two = sbase + 1 < S;
+  const int ltw = ilog2(twlen), lbits = ilog2(r);
+
+  // ---- fold + radix-4 split; sub-transform inputs 0, 1 to LDS (bit-reversed)
+  cplx<float> park[C][2];
+  if (!two)
+    fold_split_wide<NT, C, 0>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+  else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
+    fold_split_wide<NT, C, 2>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+  else
+    fold_split_wide<NT, C, 1>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+  __syncthreads();
+  fused_fft<NT, true, +1>(X, ldx, 2, r, tw, twlen);
+  // ---- results of 0, 1 to registers; inputs of 2, 3 to LDS
+  cplx<float> res[C][2];
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const int j1 = t + i * NT;
+    res[i][0] = X[pad(j1)];
+    res[i][1] = X[ldx + pad(j1)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const int pos = int(__brev(unsigned(t + i * NT)) >> (32 - lbits));
+    X[pad(pos)] = park[i][0];
+    X[ldx + pad(pos)] = park[i][1];
+  }
+  __syncthreads();
+  fused_fft<NT, true, +1>(X, ldx, 2, r, tw, twlen);
+
+  // ---- epilogue: pixels 4 j1 .. 4 j1 + 3 of each shell, one 16-byte store
+  const int64_t st = ring_start[ring];
+  int k0 = GA_T_NORMAL, kk1 = GA_T_NORMAL;
+  float l0 = 1.0f, a0 = 0.0f, l1 = 1.0f, a1 = 0.0f;
+  if (epi) {
+    k0 = epi[sbase].kind;
+    l0 = float(epi[sbase].lamda);
+    a0 = float(epi[sbase].param);
+    if (two) {
+      kk1 = epi[sbase + 1].kind;
+      l1 = float(epi[sbase + 1].lamda);
+      a1 = float(epi[sbase + 1].param);
+    }
+  }
+  float4 *out0 = reinterpret_cast<float4 *>(maps + int64_t(sbase) * npix + st);
+  float4 *out1 = reinterpret_cast<float4 *>(maps + int64_t(sbase + 1) * npix + st);
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const int j1 = t + i * NT;
+    const cplx<float> y[4] = {res[i][0], res[i][1], X[pad(j1)], X[ldx + pad(j1)]};
+    out0[j1] = float4{apply_epi<float>(y[0].re, k0, l0, a0), apply_epi<float>(y[1].re, k0, l0, a0),
+                      apply_epi<float>(y[2].re, k0, l0, a0), apply_epi<float>(y[3].re, k0, l0, a0)};
+    if (two)
+      out1[j1] = float4{apply_epi<float>(y[0].im, kk1, l1, a1), apply_epi<float>(y[1].im, kk1, l1, a1),
+                        apply_epi<float>(y[2].im, kk1, l1, a1), apply_epi<float>(y[3].im, kk1, l1, a1)};
+  }
+}
+
+template <int NT, int C>
+int launch_synth_wide(const ga_plan *p, int S, int G, const float *F, float *maps,
+                      const ga_epilogue *epi, hipStream_t stream) {
+  const int r = p->r_wide;
+  GA_REQUIRE(r == NT * C, GA_ERR_UNSUPPORTED, "wide ring kernel for r = %d launched with r = %d",
+             NT * C, r);
+  const int ldx = padded_len(r);
+  const size_t lds = size_t(2) * ldx * sizeof(cplx<float>);
+  const int nunit = (S + 1) / 2;
+  const int nblk = 8 * ((p->n_wide + 7) / 8) * nunit;
+  auto kfn = ring_synth_wide_kernel<NT, C>;
+  GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(lds)));
+  hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, p->d_rings_wide,
+                     p->n_wide, p->d_ring_start, r, p->d_tw32, p->tw_len, ldx,
+                     reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps, p->npix, epi);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 synthesis of the wide Bluestein rings (non-power-of-two r, n = 4 r >
+// lmax, L = blu_len(r) <= Lmax of the class): the fold and the radix-4 split
+// run in registers as for the wide power-of-two rings (bins k1 + r k2 of the
+// thread's k1 = t + c NT, c < C), then the four Bluestein convolutions run A
+// at a time on A LDS arrays of Lmax (A = 2: half the LDS of the four-array
+// form): the remaining split outputs and the finished results
+// wait in registers.  Launched with A = 2.  Same arithmetic, term order and passes as
+// ring_synth2_kernel with BLU.
+template <int NT, int C, int MODE>
+__device__ __forceinline__ void fold_split_wide_blu(const cplx<float> *__restrict__ Fu, int G,
+                                                    int r, int lmax,
+                                                    const float2 *__restrict__ chirp,
+                                                    cplx<float> (&y)[C][4]) {
+  const int t = threadIdx.x;
+  const int n = 4 * r;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int k1 = t + c * NT;
+    cplx<float> g[4][2];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) wide_bin_first<MODE>(Fu, G, n, lmax, k1 + k2 * r, k1 < r, g[k2]);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) wide_bin_second<MODE>(Fu, G, n, lmax, k1 + k2 * r, k1 < r, g[k2]);
+    cplx<float> h[4];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2)
+      h[k2] = cplx<float>{g[k2][0].re - g[k2][1].im, g[k2][0].im + g[k2][1].re};
+    const cplx<float> w1 = expi_q(k1 < r ? k1 : 0, n, 0.0f);
+    const cplx<float> w2 = cmul(w1, w1);
+    const cplx<float> w3 = cmul(w2, w1);
+    const cplx<float> s02 = cadd(h[0], h[2]), d02 = csub(h[0], h[2]);
+    const cplx<float> s13 = cadd(h[1], h[3]), d13 = cmuli(csub(h[1], h[3]));
+    const float2 cc = chirp[k1 < r ? k1 : 0];
+    const cplx<float> ch = {cc.x, cc.y};
+    y[c][0] = cmul(cadd(s02, s13), ch);
+    y[c][1] = cmul(cmul(cadd(d02, d13), w1), ch);
+    y[c][2] = cmul(cmul(csub(s02, s13), w2), ch);
+    y[c][3] = cmul(cmul(csub(d02, d13), w3), ch);
+    // loads of at most two k1 values in flight (registers)
+    if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int NT, int C, int A>
+__global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
+    int lmax, const int32_t *__restrict__ ring_list, int nlist,
+    const int64_t *__restrict__ ring_start, const int32_t *__restrict__ ring_n,
+    const int64_t *__restrict__ blu_off, const float2 *__restrict__ blu,
+    const float2 *__restrict__ tw, int twlen, int ldx, const cplx<float> *__restrict__ F, int S,
+    int G, int nunit, float *__restrict__ maps, int64_t npix,
+    const ga_epilogue *__restrict__ epi) {
+  extern __shared__ char smem_raw[];
+  cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
+  int li, unit;
+  if (!decode_ring_block(nlist, nunit, li, unit)) return;
+  const int ring = ring_list[li];
+  const int t = threadIdx.x;
+  const int sbase = 2 * unit;
+  const int r = ring_n[ring] >> 2;
+  const int ngrp = (S + G - 1) / G;
+  const cplx<float> *Fu = F + (int64_t(ring) * ngrp + sbase / G) * (lmax + 1) * G + sbase % G;
+  const bool two = sbase + 1 < S;
+  const float2 *__restrict__ chirp = blu + blu_off[r];
+  const float2 *__restrict__ bspec = chirp + r;
+  const int L = blu_len(r), lbits = ilog2(L);
+
+  // ---- fold + radix-4 split + input chirp, in registers
+  cplx<float> y[C][4];
+  if (!two)
+    fold_split_wide_blu<NT, C, 0>(Fu, G, r, lmax, chirp, y);
+  else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
+    fold_split_wide_blu<NT, C, 2>(Fu, G, r, lmax, chirp, y);
+  else
+    fold_split_wide_blu<NT, C, 1>(Fu, G, r, lmax, chirp, y);
+
+  // ---- the four convolutions, A at a time; results back into y
+#pragma unroll
+  for (int q0 = 0; q0 < 4; q0 += A) {
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int k1 = t + c * NT;
+        if (k1 < r) X[a * ldx + pad(k1)] = y[c][q0 + a];
+      }
+    for (int e = t; e < (L - r) * A; e += NT) {
+      const int a = e / (L - r), j = r + e - a * (L - r);
+      X[a * ldx + pad(j)] = cplx<float>{0.0f, 0.0f};
+    }
+    __syncthreads();
+    fused_fft<NT, false, -1>(X, ldx, A, L, tw, twlen);  // natural -> bit-reversed
+    for (int e = t; e < L * A; e += NT) {
+      const int a = e >> lbits, j = e & (L - 1);
+      const float2 b = bspec[j];  // stored bit-reversed
+      X[a * ldx + pad(j)] = cmul(X[a * ldx + pad(j)], cplx<float>{b.x, b.y});
+    }
+    __syncthreads();
+    fused_fft<NT, true, +1>(X, ldx, A, L, tw, twlen);  // bit-reversed -> natural
+#pragma unroll
+    for (int a = 0; a < A; ++a)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int j1 = t + c * NT;
+        if (j1 < r) y[c][q0 + a] = X[a * ldx + pad(j1)];
+      }
+    __syncthreads();
+  }
+
+  // ---- epilogue: output chirp, pixels 4 j1 .. 4 j1 + 3 of each shell
+  const int64_t st = ring_start[ring];
+  int k0 = GA_T_NORMAL, kk1 = GA_T_NORMAL;
+  float l0 = 1.0f, a0 = 0.0f, l1 = 1.0f, a1 = 0.0f;
+  if (epi) {
+    k0 = epi[sbase].kind;
+    l0 = float(epi[sbase].lamda);
+    a0 = float(epi[sbase].param);
+    if (two) {
+      kk1 = epi[sbase + 1].kind;
+      l1 = float(epi[sbase + 1].lamda);
+      a1 = float(epi[sbase + 1].param);
+    }
+  }
+  float4 *out0 = reinterpret_cast<float4 *>(maps + int64_t(sbase) * npix + st);
+  float4 *out1 = reinterpret_cast<float4 *>(maps + int64_t(sbase + 1) * npix + st);
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int j1 = t + c * NT;
+    if (j1 < r) {
+      const float2 cc = chirp[j1];
+      const cplx<float> cj = {cc.x, cc.y};
+      cplx<float> v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = cmul(y[c][q], cj);
+      out0[j1] = float4{apply_epi<float>(v[0].re, k0, l0, a0), apply_epi<float>(v[1].re, k0, l0, a0),
+                        apply_epi<float>(v[2].re, k0, l0, a0), apply_epi<float>(v[3].re, k0, l0, a0)};
+      if (two)
+        out1[j1] = float4{apply_epi<float>(v[0].im, kk1, l1, a1), apply_epi<float>(v[1].im, kk1, l1, a1),
+                          apply_epi<float>(v[2].im, kk1, l1, a1), apply_epi<float>(v[3].im, kk1, l1, a1)};
+    }
+  }
+}
+
+template <int NT, int C, int A>
+int launch_synth_wide_blu(const ga_plan *p, int S, int G, const float *F, float *maps,
+                          const ga_epilogue *epi, hipStream_t stream) {
+  GA_REQUIRE(p->max_r_bwide <= NT * C && p->max_L_bwide / 16 <= NT, GA_ERR_UNSUPPORTED,
+             "wide Bluestein kernel: r %d, L %d", p->max_r_bwide, p->max_L_bwide);
+  const int ldx = padded_len(p->max_L_bwide);
+  const size_t lds = size_t(A) * ldx * sizeof(cplx<float>);
+  GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds);
+  const int nunit = (S + 1) / 2;
+  const int nblk = 8 * ((p->n_bwide + 7) / 8) * nunit;
+  auto kfn = ring_synth_wide_blu_kernel<NT, C, A>;
+  GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(lds)));
+  hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, p->d_rings_bwide,
+                     p->n_bwide, p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32,
+                     p->d_tw32, p->tw_len, ldx, reinterpret_cast<const cplx<float> *>(F), S, G,
+                     nunit, maps, p->npix, epi);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+// both ring classes with the fused-pass kernels (the wide power-of-two rings
+// by the register-fold kernel when wide_rings); GA_ERR_UNSUPPORTED (before any
 // launch) if a class exceeds the instantiated sizes, and the caller then uses
 // the first form
 int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
-                           const ga_epilogue *epi, hipStream_t stream) {
+                           const ga_epilogue *epi, bool wide_rings, hipStream_t stream) {
   // a unit's two shells must be adjacent in the ring buffer, and the pixel
   // stores are 16-byte vectors
   if (!(G == S || G % 2 == 0)) return GA_ERR_UNSUPPORTED;
   if (reinterpret_cast<uintptr_t>(maps) & 15) return GA_ERR_UNSUPPORTED;
-  const int Np = p->max_r_pow2, Nb = p->max_L_blu, rb = p->max_r_blu;
-  if (p->n_pow2 > 0 && Np > 4096) return GA_ERR_UNSUPPORTED;
-  if (p->n_blu > 0 && Nb > 4096) return GA_ERR_UNSUPPORTED;
+  // wide rings by the register-fold kernel (unless switched off), the other
+  // power-of-two rings by the four-array kernel
+  const bool wide = wide_rings && p->n_wide > 0;
+  const int32_t *lp2 = wide ? p->d_rings_narrow : p->d_rings_pow2;
+  const int np2 = wide ? p->n_narrow : p->n_pow2;
+  const bool bwide = wide_rings && p->n_bwide > 0;
+  const int32_t *lb2 = bwide ? p->d_rings_bnarrow : p->d_rings_blu;
+  const int nb2 = bwide ? p->n_bnarrow : p->n_blu;
+  const int Np = wide ? p->max_r_narrow : p->max_r_pow2;
+  const int Nb = bwide ? p->max_L_bnarrow : p->max_L_blu;
+  const int rb = bwide ? p->max_r_bnarrow : p->max_r_blu;
+  if (np2 > 0 && Np > 4096) return GA_ERR_UNSUPPORTED;
+  if (nb2 > 0 && Nb > 4096) return GA_ERR_UNSUPPORTED;
   int rc = GA_OK;
-  if (p->n_pow2 > 0) {
-    const int32_t *l = p->d_rings_pow2;
-    const int n = p->n_pow2;
+  if (bwide) {
+    // two convolutions at a time; at most four k1 values per thread
+    const int cr = (p->max_r_bwide + 255) / 256;
+    if (cr <= 1) rc = launch_synth_wide_blu<256, 1, 2>(p, S, G, F, maps, epi, stream);
+    else if (cr <= 2) rc = launch_synth_wide_blu<256, 2, 2>(p, S, G, F, maps, epi, stream);
+    else if (cr <= 4) rc = launch_synth_wide_blu<256, 4, 2>(p, S, G, F, maps, epi, stream);
+    else rc = launch_synth_wide_blu<512, 4, 2>(p, S, G, F, maps, epi, stream);
+    if (rc) return rc;
+  }
+  if (wide) {
+    switch (p->r_wide) {
+      case 256: rc = launch_synth_wide<256, 1>(p, S, G, F, maps, epi, stream); break;
+      case 512: rc = launch_synth_wide<256, 2>(p, S, G, F, maps, epi, stream); break;
+      case 1024: rc = launch_synth_wide<256, 4>(p, S, G, F, maps, epi, stream); break;
+      case 2048: rc = launch_synth_wide<256, 8>(p, S, G, F, maps, epi, stream); break;
+      default: rc = launch_synth_wide<512, 8>(p, S, G, F, maps, epi, stream); break;
+    }
+    if (rc) return rc;
+  }
+  if (np2 > 0) {
+    const int32_t *l = lp2;
+    const int n = np2;
     // LDS per workgroup: 4 arrays of N padded complex floats (67.6 KB at N =
     // 2048, 135 KB at 4096: one workgroup per CU, so 512 threads there)
     if (Np <= 256) rc = launch_synth2<256, 1, false>(p, l, n, Np, S, G, F, maps, epi, stream);
@@ -1049,9 +1418,9 @@ int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float
     else rc = launch_synth2<512, 8, false>(p, l, n, Np, S, G, F, maps, epi, stream);
     if (rc) return rc;
   }
-  if (p->n_blu > 0) {
-    const int32_t *l = p->d_rings_blu;
-    const int n = p->n_blu;
+  if (nb2 > 0) {
+    const int32_t *l = lb2;
+    const int n = nb2;
     if (Nb <= 2048) {
       if (rb <= 256) rc = launch_synth2<256, 1, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
       else if (rb <= 512) rc = launch_synth2<256, 2, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
@@ -1151,7 +1520,7 @@ int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float 
                           const ga_epilogue *epi, hipStream_t s) {
   if (tuning().ring_path != 1) {
     // the second form needs whole units of 2P shells in one group
-    const int rc = launch_ring_synth2_f32(p, S, G, F, maps, epi, s);
+    const int rc = launch_ring_synth2_f32(p, S, G, F, maps, epi, tuning().ring_path != 2, s);
     if (rc != GA_ERR_UNSUPPORTED) return rc;
     clear_error();
   }

@@ -139,6 +139,36 @@ def test_ring_paths_agree(nside, lmax):
         assert relerr(out[2][s].cpu().numpy(), ref) < 1e-5
 
 
+@pytest.mark.parametrize("nside,lmax,S", [(256, 511, 6), (256, 700, 5), (512, 1023, 4)])
+def test_wide_ring_kernel_matches_fused_passes(nside, lmax, S):
+    """The register-fold kernel of the wide power-of-two rings (4 r > lmax)
+    agrees with the four-array fused-pass kernel (same terms, order and
+    passes; the compiler may contract products into FMAs differently, so to
+    FP32 rounding: max |difference| / RMS < 2e-6), including bins with both a
+    direct and a conjugate term (lmax > 2 r) and a lone last shell (odd S);
+    and stays within 1e-5 of FP64."""
+    from glass_amd import _lib
+
+    print(f"wide rings nside={nside} lmax={lmax} S={S}", flush=True)
+    gls = shell_spectra(S, lmax, rho=(0.6, 0.3, 0.1))
+    eng = LognormalShells(gls, nside, ncorr=3, seed=11, lmax=lmax)
+    out = {}
+    v = _lib.ctypes.c_int32()
+    _lib.call("ga_get_tuning", _lib.TUNE_RING_PATH, _lib.ctypes.byref(v))
+    try:
+        for path in (0, 2):
+            _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, path)
+            eng.run()
+            torch.cuda.synchronize()
+            out[path] = eng.maps.clone()
+    finally:
+        _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, v.value)
+    for s in range(S):
+        assert relerr(out[0][s].cpu().numpy(), out[2][s].cpu().numpy()) < 2e-6, s
+    ref = eng.fp64_reference_map(S - 1).cpu().numpy()
+    assert relerr(out[0][S - 1].cpu().numpy(), ref) < 1e-5
+
+
 @pytest.mark.parametrize("spin", [1, 2])
 @pytest.mark.parametrize("nside,lmax,S", [(8, 20, 3), (64, 150, 5), (256, 511, 4)])
 def test_spin_mfma_matches_valu_and_fp64(spin, nside, lmax, S):
