@@ -1208,6 +1208,24 @@ int launch_synth_wide(const ga_plan *p, int S, int G, const float *F, float *map
 // form): the remaining split outputs and the finished results
 // wait in registers.  Launched with A = 2.  Same arithmetic, term order and passes as
 // ring_synth2_kernel with BLU.
+// radix-4 split of the bins h[k2] = bin k1 + r k2 of a Bluestein ring and the
+// input chirp: y[q] = (sum_k2 i^(q k2) h[k2]) e^{2 pi i q k1 / n} chirp[k1]
+__device__ __forceinline__ void split_chirp(const cplx<float> (&h)[4], int k1, int n,
+                                            const float2 *__restrict__ chirp,
+                                            cplx<float> (&y)[4]) {
+  const cplx<float> w1 = expi_q(k1, n, 0.0f);
+  const cplx<float> w2 = cmul(w1, w1);
+  const cplx<float> w3 = cmul(w2, w1);
+  const cplx<float> s02 = cadd(h[0], h[2]), d02 = csub(h[0], h[2]);
+  const cplx<float> s13 = cadd(h[1], h[3]), d13 = cmuli(csub(h[1], h[3]));
+  const float2 cc = chirp[k1];
+  const cplx<float> ch = {cc.x, cc.y};
+  y[0] = cmul(cadd(s02, s13), ch);
+  y[1] = cmul(cmul(cadd(d02, d13), w1), ch);
+  y[2] = cmul(cmul(csub(s02, s13), w2), ch);
+  y[3] = cmul(cmul(csub(d02, d13), w3), ch);
+}
+
 template <int NT, int C, int MODE>
 __device__ __forceinline__ void fold_split_wide_blu(const cplx<float> *__restrict__ Fu, int G,
                                                     int r, int lmax,
@@ -1227,23 +1245,29 @@ __device__ __forceinline__ void fold_split_wide_blu(const cplx<float> *__restric
 #pragma unroll
     for (int k2 = 0; k2 < 4; ++k2)
       h[k2] = cplx<float>{g[k2][0].re - g[k2][1].im, g[k2][0].im + g[k2][1].re};
-    const cplx<float> w1 = expi_q(k1 < r ? k1 : 0, n, 0.0f);
-    const cplx<float> w2 = cmul(w1, w1);
-    const cplx<float> w3 = cmul(w2, w1);
-    const cplx<float> s02 = cadd(h[0], h[2]), d02 = csub(h[0], h[2]);
-    const cplx<float> s13 = cadd(h[1], h[3]), d13 = cmuli(csub(h[1], h[3]));
-    const float2 cc = chirp[k1 < r ? k1 : 0];
-    const cplx<float> ch = {cc.x, cc.y};
-    y[c][0] = cmul(cadd(s02, s13), ch);
-    y[c][1] = cmul(cmul(cadd(d02, d13), w1), ch);
-    y[c][2] = cmul(cmul(csub(s02, s13), w2), ch);
-    y[c][3] = cmul(cmul(csub(d02, d13), w3), ch);
+    split_chirp(h, k1 < r ? k1 : 0, n, chirp, y[c]);
     // loads of at most two k1 values in flight (registers)
     if (c % 2 == 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-template <int NT, int C, int A>
+// the same from bins already folded into LDS (bin k1 + r k2 at X[k2 ldf + pad(k1)])
+template <int NT, int C>
+__device__ __forceinline__ void split_from_lds(const cplx<float> *X, int ldf, int r,
+                                               const float2 *__restrict__ chirp,
+                                               cplx<float> (&y)[C][4]) {
+  const int n = 4 * r;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int k1 = min(int(threadIdx.x) + c * NT, r - 1);
+    cplx<float> h[4];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) h[k2] = X[k2 * ldf + pad(k1)];
+    split_chirp(h, k1, n, chirp, y[c]);
+  }
+}
+
+template <int NT, int C, int A, bool LDSFOLD>
 __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
     int lmax, const int32_t *__restrict__ ring_list, int nlist,
     const int64_t *__restrict__ ring_start, const int32_t *__restrict__ ring_n,
@@ -1266,14 +1290,28 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
   const float2 *__restrict__ bspec = chirp + r;
   const int L = blu_len(r), lbits = ilog2(L);
 
-  // ---- fold + radix-4 split + input chirp, in registers
+  // ---- fold + radix-4 split + input chirp, in registers (LDSFOLD: the fold
+  // of a ring with n <= lmax goes through four packed LDS arrays first)
   cplx<float> y[C][4];
-  if (!two)
-    fold_split_wide_blu<NT, C, 0>(Fu, G, r, lmax, chirp, y);
-  else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
-    fold_split_wide_blu<NT, C, 2>(Fu, G, r, lmax, chirp, y);
-  else
-    fold_split_wide_blu<NT, C, 1>(Fu, G, r, lmax, chirp, y);
+  if (LDSFOLD) {
+    const int n = 4 * r, ldf = padded_len(r);
+    if (!two)
+      fold_bins<NT, 0>(Fu, G, n, r, lmax, 0, ldf, X);
+    else if (G % 2 == 0)
+      fold_bins<NT, 2>(Fu, G, n, r, lmax, 0, ldf, X);
+    else
+      fold_bins<NT, 1>(Fu, G, n, r, lmax, 0, ldf, X);
+    __syncthreads();
+    split_from_lds<NT, C>(X, ldf, r, chirp, y);
+    __syncthreads();
+  } else {
+    if (!two)
+      fold_split_wide_blu<NT, C, 0>(Fu, G, r, lmax, chirp, y);
+    else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
+      fold_split_wide_blu<NT, C, 2>(Fu, G, r, lmax, chirp, y);
+    else
+      fold_split_wide_blu<NT, C, 1>(Fu, G, r, lmax, chirp, y);
+  }
 
   // ---- the four convolutions, A at a time; results back into y
 #pragma unroll
@@ -1342,25 +1380,50 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
   }
 }
 
-template <int NT, int C, int A>
-int launch_synth_wide_blu(const ga_plan *p, int S, int G, const float *F, float *maps,
-                          const ga_epilogue *epi, hipStream_t stream) {
-  GA_REQUIRE(p->max_r_bwide <= NT * C && p->max_L_bwide / 16 <= NT, GA_ERR_UNSUPPORTED,
-             "wide Bluestein kernel: r %d, L %d", p->max_r_bwide, p->max_L_bwide);
-  const int ldx = padded_len(p->max_L_bwide);
-  const size_t lds = size_t(A) * ldx * sizeof(cplx<float>);
+// one Bluestein class (list, count, class maxima of r and L) by the
+// register-split kernel; LDSFOLD for rings with n <= lmax
+template <int NT, int C, int A, bool LDSFOLD>
+int launch_synth_wide_blu(const ga_plan *p, const int32_t *list, int nlist, int max_r, int max_L,
+                          int S, int G, const float *F, float *maps, const ga_epilogue *epi,
+                          hipStream_t stream) {
+  GA_REQUIRE(max_r <= NT * C, GA_ERR_UNSUPPORTED, "Bluestein kernel: r %d above %d", max_r,
+             NT * C);
+  const int ldx = padded_len(max_L);
+  size_t ent = size_t(A) * ldx;
+  // the LDS fold: four packed arrays of r, and 2 NT partial sums of small rings
+  if (LDSFOLD) ent = std::max({ent, size_t(4) * padded_len(max_r), size_t(2) * NT});
+  const size_t lds = ent * sizeof(cplx<float>);
   GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring synthesis needs %zu bytes of LDS", lds);
   const int nunit = (S + 1) / 2;
-  const int nblk = 8 * ((p->n_bwide + 7) / 8) * nunit;
-  auto kfn = ring_synth_wide_blu_kernel<NT, C, A>;
+  const int nblk = 8 * ((nlist + 7) / 8) * nunit;
+  auto kfn = ring_synth_wide_blu_kernel<NT, C, A, LDSFOLD>;
   GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    int(lds)));
-  hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, p->d_rings_bwide,
-                     p->n_bwide, p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32,
-                     p->d_tw32, p->tw_len, ldx, reinterpret_cast<const cplx<float> *>(F), S, G,
-                     nunit, maps, p->npix, epi);
+  hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, list, nlist,
+                     p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32, p->d_tw32,
+                     p->tw_len, ldx, reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps,
+                     p->npix, epi);
   GA_LAUNCH_CHECK();
   return GA_OK;
+}
+
+template <bool LDSFOLD>
+int launch_blu_class(const ga_plan *p, const int32_t *list, int nlist, int max_r, int max_L,
+                     int S, int G, const float *F, float *maps, const ga_epilogue *epi,
+                     hipStream_t stream) {
+  // two convolutions at a time; at most four k1 values per thread
+  const int cr = (max_r + 255) / 256;
+  if (cr <= 1)
+    return launch_synth_wide_blu<256, 1, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
+                                                     epi, stream);
+  if (cr <= 2)
+    return launch_synth_wide_blu<256, 2, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
+                                                     epi, stream);
+  if (cr <= 4)
+    return launch_synth_wide_blu<256, 4, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
+                                                     epi, stream);
+  return launch_synth_wide_blu<512, 4, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
+                                                   epi, stream);
 }
 
 // both ring classes with the fused-pass kernels (the wide power-of-two rings
@@ -1378,22 +1441,25 @@ int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float
   const bool wide = wide_rings && p->n_wide > 0;
   const int32_t *lp2 = wide ? p->d_rings_narrow : p->d_rings_pow2;
   const int np2 = wide ? p->n_narrow : p->n_pow2;
-  const bool bwide = wide_rings && p->n_bwide > 0;
-  const int32_t *lb2 = bwide ? p->d_rings_bnarrow : p->d_rings_blu;
-  const int nb2 = bwide ? p->n_bnarrow : p->n_blu;
+  // Bluestein rings: with wide_rings by the register-split kernel, the wide
+  // class with the fold in registers, the rest with the fold through LDS
+  // (r <= 2048, L <= 4096); otherwise all by the four-array kernel
+  const bool bsplit = wide_rings && p->max_r_bnarrow <= 2048 && p->max_L_bnarrow <= 4096;
+  const int32_t *lb2 = p->d_rings_blu;
+  const int nb2 = bsplit ? 0 : p->n_blu;
   const int Np = wide ? p->max_r_narrow : p->max_r_pow2;
-  const int Nb = bwide ? p->max_L_bnarrow : p->max_L_blu;
-  const int rb = bwide ? p->max_r_bnarrow : p->max_r_blu;
+  const int Nb = p->max_L_blu, rb = p->max_r_blu;
   if (np2 > 0 && Np > 4096) return GA_ERR_UNSUPPORTED;
   if (nb2 > 0 && Nb > 4096) return GA_ERR_UNSUPPORTED;
   int rc = GA_OK;
-  if (bwide) {
-    // two convolutions at a time; at most four k1 values per thread
-    const int cr = (p->max_r_bwide + 255) / 256;
-    if (cr <= 1) rc = launch_synth_wide_blu<256, 1, 2>(p, S, G, F, maps, epi, stream);
-    else if (cr <= 2) rc = launch_synth_wide_blu<256, 2, 2>(p, S, G, F, maps, epi, stream);
-    else if (cr <= 4) rc = launch_synth_wide_blu<256, 4, 2>(p, S, G, F, maps, epi, stream);
-    else rc = launch_synth_wide_blu<512, 4, 2>(p, S, G, F, maps, epi, stream);
+  if (bsplit && p->n_bwide > 0) {
+    rc = launch_blu_class<false>(p, p->d_rings_bwide, p->n_bwide, p->max_r_bwide, p->max_L_bwide,
+                                 S, G, F, maps, epi, stream);
+    if (rc) return rc;
+  }
+  if (bsplit && p->n_bnarrow > 0) {
+    rc = launch_blu_class<true>(p, p->d_rings_bnarrow, p->n_bnarrow, p->max_r_bnarrow,
+                                p->max_L_bnarrow, S, G, F, maps, epi, stream);
     if (rc) return rc;
   }
   if (wide) {

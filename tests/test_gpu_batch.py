@@ -113,9 +113,10 @@ def test_from_convergence_fp32_batch_agrees_with_module_function():
 
 @pytest.mark.parametrize("nside,lmax", [(16, 40), (64, 128), (128, 255), (256, 512)])
 def test_ring_paths_agree(nside, lmax):
-    """Both FP32 ring-synthesis forms (radix-4 LDS stages / fused radix-2^K
-    passes, power-of-two and Bluestein rings) give the same maps, and both
-    are within 1e-5 of the FP64 path."""
+    """The FP32 ring-synthesis forms (radix-4 LDS stages / fused radix-2^K
+    passes / the default: fused passes with the register-split kernels of the
+    wide and Bluestein rings; power-of-two and Bluestein rings) give the same
+    maps, and all are within 1e-5 of the FP64 path."""
     from glass_amd import _lib
 
     S = 6
@@ -125,7 +126,7 @@ def test_ring_paths_agree(nside, lmax):
     v = _lib.ctypes.c_int32()
     _lib.call("ga_get_tuning", _lib.TUNE_RING_PATH, _lib.ctypes.byref(v))
     try:
-        for path in (1, 2):
+        for path in (0, 1, 2):
             _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, path)
             eng.run()
             torch.cuda.synchronize()
@@ -133,10 +134,12 @@ def test_ring_paths_agree(nside, lmax):
     finally:
         _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, v.value)
     for s in range(S):
-        assert relerr(out[2][s].cpu().numpy(), out[1][s].cpu().numpy()) < 2e-6, s
+        for path in (0, 2):
+            assert relerr(out[path][s].cpu().numpy(), out[1][s].cpu().numpy()) < 2e-6, (path, s)
     for s in (0, S - 1):
         ref = eng.fp64_reference_map(s).cpu().numpy()
-        assert relerr(out[2][s].cpu().numpy(), ref) < 1e-5
+        for path in (0, 2):
+            assert relerr(out[path][s].cpu().numpy(), ref) < 1e-5, (path, s)
 
 
 @pytest.mark.parametrize("nside,lmax,S", [(256, 511, 6), (256, 700, 5), (512, 1023, 4)])
