@@ -751,6 +751,81 @@ __device__ __forceinline__ void fused_fft(cplx<float> *X, int ldx, int A, int N,
   }
 }
 
+// The middle of a Bluestein convolution as one pass: the last forward DIF
+// pass (K stages, half-spans 2^(K-1) .. 1), the product with the chirp
+// spectrum (stored bit-reversed, i.e. in the order the DIF leaves the
+// values) and the first inverse DIT pass (half-spans 1 .. 2^(K-1)) act on the
+// same groups of 2^K consecutive elements, whose twiddles are the constant
+// rotations alone; one LDS round trip instead of three.
+template <int K, int NT>
+__device__ __forceinline__ void mid_pass(cplx<float> *X, int ldx, int A, int L,
+                                         const float2 *__restrict__ bspec) {
+  constexpr int R = 1 << K;
+  const int lg = ilog2(L) - K;
+  const int total = A << lg;
+  for (int gg = threadIdx.x; gg < total; gg += NT) {
+    const int a = gg >> lg, g = gg & ((1 << lg) - 1);
+    const int base = g << K;
+    cplx<float> *Y = X + a * ldx + pad(base);  // a group never straddles a pad
+    cplx<float> v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = Y[q];
+#pragma unroll
+    for (int s = K - 1; s >= 0; --s) {
+      const int d = 1 << s;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (q & d) continue;
+        const cplx<float> tq = rot16<-1>(cplx<float>{1.0f, 0.0f}, (q & (d - 1)) * (16 >> (s + 1)));
+        const cplx<float> x = v[q], y = v[q + d];
+        v[q] = cadd(x, y);
+        v[q + d] = cmul(csub(x, y), tq);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const float2 b = bspec[base + q];
+      v[q] = cmul(v[q], cplx<float>{b.x, b.y});
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const int d = 1 << s;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (q & d) continue;
+        const cplx<float> tq = rot16<1>(cplx<float>{1.0f, 0.0f}, (q & (d - 1)) * (16 >> (s + 1)));
+        const cplx<float> y = cmul(v[q + d], tq);
+        const cplx<float> x = v[q];
+        v[q] = cadd(x, y);
+        v[q + d] = csub(x, y);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) Y[q] = v[q];
+  }
+  __syncthreads();
+}
+
+// Bluestein convolution of A arrays of length L = 2^lN in place: forward DIF
+// passes (K = 4, half-spans down to 2^(l0)), the middle pass (K = l0 <= 4,
+// with the chirp spectrum), inverse DIT passes mirroring the forward ones;
+// natural order in and out.
+template <int NT>
+__device__ __forceinline__ void bluestein_conv(cplx<float> *X, int ldx, int A, int L,
+                                               const float2 *__restrict__ tw, int twlen,
+                                               const float2 *__restrict__ bspec) {
+  const int lN = ilog2(L), ltw = ilog2(twlen);
+  int l = lN;
+  for (; l > 4; l -= 4) fused_pass<4, NT, false, -1>(X, ldx, A, L, 1 << (l - 4), tw, ltw);
+  switch (l) {
+    case 4: mid_pass<4, NT>(X, ldx, A, L, bspec); break;
+    case 3: mid_pass<3, NT>(X, ldx, A, L, bspec); break;
+    case 2: mid_pass<2, NT>(X, ldx, A, L, bspec); break;
+    default: mid_pass<1, NT>(X, ldx, A, L, bspec); break;
+  }
+  for (int lo = l; lo < lN; lo += 4) fused_pass<4, NT, true, 1>(X, ldx, A, L, 1 << lo, tw, ltw);
+}
+
 // The aliasing fold of one ring's coefficients into its n = 4 r bins, for a
 // unit of two shells (h = G(s) + i G(s + 1)), written to the bin slots of X
 // (bin k = k1 + r k2 at X[k2 ldx + pad(pos(k1))], pos = bit-reversed k1 on
@@ -1328,14 +1403,7 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
       X[a * ldx + pad(j)] = cplx<float>{0.0f, 0.0f};
     }
     __syncthreads();
-    fused_fft<NT, false, -1>(X, ldx, A, L, tw, twlen);  // natural -> bit-reversed
-    for (int e = t; e < L * A; e += NT) {
-      const int a = e >> lbits, j = e & (L - 1);
-      const float2 b = bspec[j];  // stored bit-reversed
-      X[a * ldx + pad(j)] = cmul(X[a * ldx + pad(j)], cplx<float>{b.x, b.y});
-    }
-    __syncthreads();
-    fused_fft<NT, true, +1>(X, ldx, A, L, tw, twlen);  // bit-reversed -> natural
+    bluestein_conv<NT>(X, ldx, A, L, tw, twlen, bspec);
 #pragma unroll
     for (int a = 0; a < A; ++a)
 #pragma unroll
