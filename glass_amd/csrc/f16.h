@@ -39,15 +39,17 @@ __device__ __forceinline__ float sub_half_hi(float a, uint32_t h) {
 
 // x = hi + lo for two values at once: hi rounded toward zero (one packed
 // conversion; its error is captured exactly by the residual), lo = the FP32
-// residual rounded to nearest.
+// residual rounded to nearest, formed and rounded to FP16 by v_fma_mixlo_f16 /
+// v_fma_mixhi_f16 (a * 1 - h in FP32, written to one half of lo): three
+// instructions per pair.
 __device__ __forceinline__ void split_pair(float a, float b, uint32_t &hi, uint32_t &lo) {
   const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
   hi = __builtin_bit_cast(uint32_t, h);
-  const float ra = sub_half_lo(a, hi), rb = sub_half_hi(b, hi);
-  f16x2 l;
-  l[0] = (_Float16)ra;
-  l[1] = (_Float16)rb;
-  lo = bits_h2(l);
+  uint32_t l;  // the high half written by mixlo is garbage until mixhi overwrites it
+  asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(l) : "v"(a), "v"(hi));
+  asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(l)
+      : "v"(b), "v"(hi));
+  lo = l;
 }
 
 // A operands of one 32-row slab from a lane's packed values.  ph / pl
