@@ -232,24 +232,32 @@ __device__ __forceinline__ void slab_values(const double2 *__restrict__ cb, doub
 }
 
 // The dead leading slabs of a wave (see the synthesis kernel); returns the
-// first slab to run in full.  cb[2][kSlab] is the wave's coefficient double
-// buffer (slab s in cb[s & 1]); cpre carries the prefetched pair.
+// first slab to run in full.  A slab is dead when every lane either starts it
+// scaled (k < 0: all its values are below 2^-300) or keeps |lambda 2^8| below
+// 2^-25 throughout, where both FP16 halves of the split are exactly 0 (the
+// high half rounds toward zero below 2^-24, the low half to nearest): the
+// slab's A operands would be all zero and its MFMAs would add nothing.
+// cb[2][kSlab] is the wave's coefficient double buffer (slab s in
+// cb[s & 1]); cpre carries the prefetched pair.
 __device__ __forceinline__ int skip_dead_slabs(double2 (*cb)[kSlab],
                                                const double2 *__restrict__ co, int lastc,
                                                int lane, double x, int m, int lmax, int nsl,
                                                double &p0, double &p1, int &k, double2 &cpre) {
+  constexpr double kTiny = 0x1p-25;
   int s = 0;
-  while (s < nsl && m + kSlab * (s + 1) - 1 <= lmax && wave_all(k < 0)) {
+  while (s < nsl && m + kSlab * (s + 1) - 1 <= lmax && wave_all(k < 0 || fabs(p1) < kTiny)) {
     const int cur = s & 1;
     cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
     const double q0 = p0, q1 = p1;
     const int kq = k;
+    double mx = 0.0;
 #pragma unroll 8
     for (int j = 0; j < kSlab; ++j) {
       const double2 c = cb[cur][j];
       const double pn = fma(c.x * x, p1, -(c.y * p0));
       p0 = p1;
       p1 = pn;
+      mx = fmax(mx, fabs(pn));
     }
     if (k < 0 && fabs(p1) > 0x1p300) {
       p1 *= 0x1p-600;
@@ -258,7 +266,7 @@ __device__ __forceinline__ int skip_dead_slabs(double2 (*cb)[kSlab],
     }
     if (lane < kSlab) cb[cur ^ 1][lane] = cpre;
     __builtin_amdgcn_wave_barrier();
-    if (wave_any(k == 0)) {  // this slab has values: undo it
+    if (!wave_all(kq < 0 || mx < kTiny)) {  // this slab has values: undo it
       p0 = q0;
       p1 = q1;
       k = kq;
@@ -388,13 +396,13 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   if (lane < kSlab) cbuf[wave][0][lane] = cpre;
   __builtin_amdgcn_wave_barrier();
 
-  // Leading slabs in which every lane stays below the FP32 range (k < 0 from
-  // start to end: near the poles at large m) contribute nothing: only the
-  // recurrence advances, without values, splitting, B loads or MFMAs, and
-  // with one rescaling check per slab (a slab grows |lambda| by far less
-  // than the 2^700 above the threshold; power-of-two rescaling is exact, so
-  // the state matches the per-group checks bit for bit).  The slab in which
-  // some lane reaches k = 0 is undone and run again by the full loop.
+  // Leading slabs in which every lane stays below the FP16 range of the split
+  // (near the poles at large m) contribute nothing: only the recurrence
+  // advances, without values, splitting, B loads or MFMAs, and with one
+  // rescaling check per slab (a slab grows |lambda| by far less than the
+  // 2^700 above the threshold; power-of-two rescaling is exact, so the state
+  // matches the per-group checks bit for bit).  The slab in which some lane
+  // reaches FP16 range is undone and run again by the full loop.
   int s = skip_dead_slabs(cbuf[wave], co, lastc, lane, x, m, lmax, nsl, p0, p1, k, cpre);
 
   for (; s < nsl; ++s) {
