@@ -139,7 +139,50 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
   _Float16(*lw)[2][32][64] = lam[wave];  // [hi|lo][p][l'][pair]
 
   const int nsl = (lmax - m + kASlab) / kASlab;
-  for (int s = 0; s < nsl; ++s) {
+  // Leading slabs in which every lane of the workgroup either starts scaled
+  // (k < 0: values below 2^-300) or keeps |lambda 2^8| below 2^-25, where both
+  // FP16 halves of the split are exactly 0, give partial sums of exactly 0:
+  // only the recurrence advances (one rescaling check per slab; power-of-two
+  // rescaling is exact, so the state matches the per-group checks bit for
+  // bit) and the slab's rows of the partial are written as 0.  The slab in
+  // which some lane reaches FP16 range is undone and run by the full loop.
+  int s0 = 0;
+  while (s0 < nsl && m + kASlab * (s0 + 1) - 1 <= lmax) {
+    const int cur = s0 & 1;
+    cpre = co[min(kASlab * (s0 + 1) + lane, lastc)];
+    const double q0 = p0, q1 = p1;
+    const int kq = k;
+    double mx = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < kASlab; ++j) {
+      const double2 c = cbuf[wave][cur][j];
+      const double pn = fma(c.x * x, p1, -(c.y * p0));
+      p0 = p1;
+      p1 = pn;
+      mx = fmax(mx, fabs(pn));
+    }
+    if (k < 0 && fabs(p1) > 0x1p300) {
+      p1 *= 0x1p-600;
+      p0 *= 0x1p-600;
+      ++k;
+    }
+    cbuf[wave][cur ^ 1][lane] = cpre;
+    __builtin_amdgcn_wave_barrier();
+    if (!__syncthreads_and(kq < 0 || mx < 0x1p-25)) {  // values in this slab: undo it
+      p0 = q0;
+      p1 = q1;
+      k = kq;
+      break;
+    }
+    const int l0 = m + kASlab * s0;
+    for (int e = threadIdx.x; e < kASlab * 32 * NT; e += 64 * kAWaves) {
+      const int l = l0 + e / (32 * NT);
+      const int col = cg * NT * 32 + e % (32 * NT);
+      if (col < ncols) partial[(int64_t(rg) * nalm + base + (l - m)) * ncols + col] = 0.0f;
+    }
+    ++s0;
+  }
+  for (int s = s0; s < nsl; ++s) {
     const int l0 = m + kASlab * s;
     const int cur = s & 1;
     cpre = co[min(kASlab * (s + 1) + lane, lastc)];
