@@ -125,6 +125,7 @@ static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter
     const T *src = map;
     if (it > 0) {
       // map_tmp = map - S(alm)
+      bool subtract = true;
       if (sizeof(T) == 8) {
         rc = launch_legendre_synth_f64(p, (const double *)alm, (double *)F, s);
         if (rc) return rc;
@@ -132,13 +133,22 @@ static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter
       } else {
         rc = launch_legendre_synth_batch(p, S, (const float *)alm, ws, s);
         if (rc) return rc;
-        rc = launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp, nullptr,
-                                   s);
+        // the subtraction fused into the ring stage's store where it can be
+        rc = launch_ring_resid_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp,
+                                   (const float *)map, s);
+        if (rc == GA_OK) {
+          subtract = false;
+        } else if (rc == GA_ERR_UNSUPPORTED) {
+          rc = launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp,
+                                     nullptr, s);
+        }
       }
       if (rc) return rc;
-      hipLaunchKernelGGL(residual_kernel<T>, dim3(unsigned((npx + 255) / 256)), dim3(256), 0, s,
-                         npx, map, map_tmp);
-      GA_LAUNCH_CHECK();
+      if (subtract) {
+        hipLaunchKernelGGL(residual_kernel<T>, dim3(unsigned((npx + 255) / 256)), dim3(256), 0,
+                           s, npx, map, map_tmp);
+        GA_LAUNCH_CHECK();
+      }
       src = map_tmp;
     }
     if (sizeof(T) == 8) {

@@ -40,6 +40,10 @@ constexpr int kMfmaGroup = 4;  // shells per group in the MFMA stage's ring buff
 // Ring stage of analysis: maps [S][npix] -> F [ring][m][S] (weighted, phase removed)
 int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s);
 int launch_ring_anal_f32(const ga_plan *p, int S, const float *maps, float *F, hipStream_t s);
+// out = ref - FP32 ring synthesis of F (fused-pass forms only; GA_ERR_UNSUPPORTED
+// with nothing launched otherwise)
+int launch_ring_resid_f32(const ga_plan *p, int S, int G, const float *F, float *out,
+                          const float *ref, hipStream_t s);
 
 // Legendre stage of analysis: F [ring][m][S] -> alm [idx][S] (+= when accumulate)
 // partial: anal_partial_slots(p) * nalm * S complex scratch

@@ -288,6 +288,15 @@ __device__ __forceinline__ T apply_epi(T x, int kind, T lam, T par) {
   return x;
 }
 
+// ref - v for the fused residual of the analysis iterations (ref = the input
+// maps at the same pixels), v unchanged without ref
+__device__ __forceinline__ float4 resid4(const float *__restrict__ ref, int64_t off, int j1,
+                                         float4 v) {
+  if (!ref) return v;
+  const float4 r = reinterpret_cast<const float4 *>(ref + off)[j1];
+  return float4{r.x - v.x, r.y - v.y, r.z - v.z, r.w - v.w};
+}
+
 // grid: 8 * ceil(nrings / 8) * npp blocks, XCD-aware: all pairs of one ring on
 // one XCD (block b runs on XCD b % 8), consecutive in that XCD's queue.
 __device__ __forceinline__ bool decode_ring_block(int nrings, int npp, int &ring, int &pair) {
@@ -963,7 +972,7 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
     const int32_t *__restrict__ ring_n, const int64_t *__restrict__ blu_off,
     const float2 *__restrict__ blu, const float2 *__restrict__ tw, int twlen, int ldx,
     const cplx<float> *__restrict__ F, int S, int G, int nunit, float *__restrict__ maps,
-    int64_t npix, const ga_epilogue *__restrict__ epi) {
+    int64_t npix, const ga_epilogue *__restrict__ epi, const float *__restrict__ ref) {
   extern __shared__ char smem_raw[];
   cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
   int li, unit;
@@ -1070,17 +1079,18 @@ __global__ __launch_bounds__(NT) void ring_synth2_kernel(
       y[q] = X[q * ldx + pad(j1)];
       if (BLU) y[q] = cmul(y[q], cj);
     }
-    out0[j1] = float4{apply_epi<float>(y[0].re, k0, l0, a0), apply_epi<float>(y[1].re, k0, l0, a0),
-                      apply_epi<float>(y[2].re, k0, l0, a0), apply_epi<float>(y[3].re, k0, l0, a0)};
+    out0[j1] = resid4(ref, int64_t(sbase) * npix + st, j1, float4{apply_epi<float>(y[0].re, k0, l0, a0), apply_epi<float>(y[1].re, k0, l0, a0),
+                      apply_epi<float>(y[2].re, k0, l0, a0), apply_epi<float>(y[3].re, k0, l0, a0)});
     if (two)
-      out1[j1] = float4{apply_epi<float>(y[0].im, kk1, l1, a1), apply_epi<float>(y[1].im, kk1, l1, a1),
-                        apply_epi<float>(y[2].im, kk1, l1, a1), apply_epi<float>(y[3].im, kk1, l1, a1)};
+      out1[j1] = resid4(ref, int64_t(sbase + 1) * npix + st, j1, float4{apply_epi<float>(y[0].im, kk1, l1, a1), apply_epi<float>(y[1].im, kk1, l1, a1),
+                        apply_epi<float>(y[2].im, kk1, l1, a1), apply_epi<float>(y[3].im, kk1, l1, a1)});
   }
 }
 
 template <int NT, int MAXC, bool BLU>
 int launch_synth2(const ga_plan *p, const int32_t *list, int nlist, int N, int S, int G,
-                  const float *F, float *maps, const ga_epilogue *epi, hipStream_t stream) {
+                  const float *F, float *maps, const ga_epilogue *epi, const float *ref,
+                  hipStream_t stream) {
   if (nlist == 0) return GA_OK;
   const int ldx = padded_len(N);
   // four sub-transform arrays; the fold's partial sums need 2 NT entries
@@ -1094,7 +1104,7 @@ int launch_synth2(const ga_plan *p, const int32_t *list, int nlist, int N, int S
   hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, list, nlist,
                      p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32, p->d_tw32,
                      p->tw_len, ldx, reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps,
-                     p->npix, epi);
+                     p->npix, epi, ref);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
@@ -1185,7 +1195,8 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_kernel(
     int lmax, const int32_t *__restrict__ ring_list, int nlist,
     const int64_t *__restrict__ ring_start, int r, const float2 *__restrict__ tw, int twlen,
     int ldx, const cplx<float> *__restrict__ F, int S, int G, int nunit,
-    float *__restrict__ maps, int64_t npix, const ga_epilogue *__restrict__ epi) {
+    float *__restrict__ maps, int64_t npix, const ga_epilogue *__restrict__ epi,
+    const float *__restrict__ ref) {
   extern __shared__ char smem_raw[];
   cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
   int li, unit;
@@ -1246,17 +1257,17 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_kernel(
   for (int i = 0; i < C; ++i) {
     const int j1 = t + i * NT;
     const cplx<float> y[4] = {res[i][0], res[i][1], X[pad(j1)], X[ldx + pad(j1)]};
-    out0[j1] = float4{apply_epi<float>(y[0].re, k0, l0, a0), apply_epi<float>(y[1].re, k0, l0, a0),
-                      apply_epi<float>(y[2].re, k0, l0, a0), apply_epi<float>(y[3].re, k0, l0, a0)};
+    out0[j1] = resid4(ref, int64_t(sbase) * npix + st, j1, float4{apply_epi<float>(y[0].re, k0, l0, a0), apply_epi<float>(y[1].re, k0, l0, a0),
+                      apply_epi<float>(y[2].re, k0, l0, a0), apply_epi<float>(y[3].re, k0, l0, a0)});
     if (two)
-      out1[j1] = float4{apply_epi<float>(y[0].im, kk1, l1, a1), apply_epi<float>(y[1].im, kk1, l1, a1),
-                        apply_epi<float>(y[2].im, kk1, l1, a1), apply_epi<float>(y[3].im, kk1, l1, a1)};
+      out1[j1] = resid4(ref, int64_t(sbase + 1) * npix + st, j1, float4{apply_epi<float>(y[0].im, kk1, l1, a1), apply_epi<float>(y[1].im, kk1, l1, a1),
+                        apply_epi<float>(y[2].im, kk1, l1, a1), apply_epi<float>(y[3].im, kk1, l1, a1)});
   }
 }
 
 template <int NT, int C>
 int launch_synth_wide(const ga_plan *p, int S, int G, const float *F, float *maps,
-                      const ga_epilogue *epi, hipStream_t stream) {
+                      const ga_epilogue *epi, const float *ref, hipStream_t stream) {
   const int r = p->r_wide;
   GA_REQUIRE(r == NT * C, GA_ERR_UNSUPPORTED, "wide ring kernel for r = %d launched with r = %d",
              NT * C, r);
@@ -1269,7 +1280,8 @@ int launch_synth_wide(const ga_plan *p, int S, int G, const float *F, float *map
                                    int(lds)));
   hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, p->d_rings_wide,
                      p->n_wide, p->d_ring_start, r, p->d_tw32, p->tw_len, ldx,
-                     reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps, p->npix, epi);
+                     reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps, p->npix, epi,
+                     ref);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
@@ -1349,7 +1361,7 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
     const int64_t *__restrict__ blu_off, const float2 *__restrict__ blu,
     const float2 *__restrict__ tw, int twlen, int ldx, const cplx<float> *__restrict__ F, int S,
     int G, int nunit, float *__restrict__ maps, int64_t npix,
-    const ga_epilogue *__restrict__ epi) {
+    const ga_epilogue *__restrict__ epi, const float *__restrict__ ref) {
   extern __shared__ char smem_raw[];
   cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
   int li, unit;
@@ -1439,11 +1451,11 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
       cplx<float> v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = cmul(y[c][q], cj);
-      out0[j1] = float4{apply_epi<float>(v[0].re, k0, l0, a0), apply_epi<float>(v[1].re, k0, l0, a0),
-                        apply_epi<float>(v[2].re, k0, l0, a0), apply_epi<float>(v[3].re, k0, l0, a0)};
+      out0[j1] = resid4(ref, int64_t(sbase) * npix + st, j1, float4{apply_epi<float>(v[0].re, k0, l0, a0), apply_epi<float>(v[1].re, k0, l0, a0),
+                        apply_epi<float>(v[2].re, k0, l0, a0), apply_epi<float>(v[3].re, k0, l0, a0)});
       if (two)
-        out1[j1] = float4{apply_epi<float>(v[0].im, kk1, l1, a1), apply_epi<float>(v[1].im, kk1, l1, a1),
-                          apply_epi<float>(v[2].im, kk1, l1, a1), apply_epi<float>(v[3].im, kk1, l1, a1)};
+        out1[j1] = resid4(ref, int64_t(sbase + 1) * npix + st, j1, float4{apply_epi<float>(v[0].im, kk1, l1, a1), apply_epi<float>(v[1].im, kk1, l1, a1),
+                          apply_epi<float>(v[2].im, kk1, l1, a1), apply_epi<float>(v[3].im, kk1, l1, a1)});
     }
   }
 }
@@ -1453,7 +1465,7 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
 template <int NT, int C, int A, bool LDSFOLD>
 int launch_synth_wide_blu(const ga_plan *p, const int32_t *list, int nlist, int max_r, int max_L,
                           int S, int G, const float *F, float *maps, const ga_epilogue *epi,
-                          hipStream_t stream) {
+                          const float *ref, hipStream_t stream) {
   GA_REQUIRE(max_r <= NT * C, GA_ERR_UNSUPPORTED, "Bluestein kernel: r %d above %d", max_r,
              NT * C);
   const int ldx = padded_len(max_L);
@@ -1470,7 +1482,7 @@ int launch_synth_wide_blu(const ga_plan *p, const int32_t *list, int nlist, int 
   hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, list, nlist,
                      p->d_ring_start, p->d_ring_n, p->d_blu_off, p->d_blu32, p->d_tw32,
                      p->tw_len, ldx, reinterpret_cast<const cplx<float> *>(F), S, G, nunit, maps,
-                     p->npix, epi);
+                     p->npix, epi, ref);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
@@ -1478,20 +1490,20 @@ int launch_synth_wide_blu(const ga_plan *p, const int32_t *list, int nlist, int 
 template <bool LDSFOLD>
 int launch_blu_class(const ga_plan *p, const int32_t *list, int nlist, int max_r, int max_L,
                      int S, int G, const float *F, float *maps, const ga_epilogue *epi,
-                     hipStream_t stream) {
+                     const float *ref, hipStream_t stream) {
   // two convolutions at a time; at most four k1 values per thread
   const int cr = (max_r + 255) / 256;
   if (cr <= 1)
     return launch_synth_wide_blu<256, 1, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
-                                                     epi, stream);
+                                                     epi, ref, stream);
   if (cr <= 2)
     return launch_synth_wide_blu<256, 2, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
-                                                     epi, stream);
+                                                     epi, ref, stream);
   if (cr <= 4)
     return launch_synth_wide_blu<256, 4, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
-                                                     epi, stream);
+                                                     epi, ref, stream);
   return launch_synth_wide_blu<512, 4, 2, LDSFOLD>(p, list, nlist, max_r, max_L, S, G, F, maps,
-                                                   epi, stream);
+                                                   epi, ref, stream);
 }
 
 // both ring classes with the fused-pass kernels (the wide power-of-two rings
@@ -1499,11 +1511,13 @@ int launch_blu_class(const ga_plan *p, const int32_t *list, int nlist, int max_r
 // launch) if a class exceeds the instantiated sizes, and the caller then uses
 // the first form
 int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
-                           const ga_epilogue *epi, bool wide_rings, hipStream_t stream) {
+                           const ga_epilogue *epi, bool wide_rings, hipStream_t stream,
+                           const float *ref = nullptr) {
   // a unit's two shells must be adjacent in the ring buffer, and the pixel
   // stores are 16-byte vectors
   if (!(G == S || G % 2 == 0)) return GA_ERR_UNSUPPORTED;
   if (reinterpret_cast<uintptr_t>(maps) & 15) return GA_ERR_UNSUPPORTED;
+  if (reinterpret_cast<uintptr_t>(ref) & 15) return GA_ERR_UNSUPPORTED;
   // wide rings by the register-fold kernel (unless switched off), the other
   // power-of-two rings by the four-array kernel
   const bool wide = wide_rings && p->n_wide > 0;
@@ -1522,21 +1536,21 @@ int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float
   int rc = GA_OK;
   if (bsplit && p->n_bwide > 0) {
     rc = launch_blu_class<false>(p, p->d_rings_bwide, p->n_bwide, p->max_r_bwide, p->max_L_bwide,
-                                 S, G, F, maps, epi, stream);
+                                 S, G, F, maps, epi, ref, stream);
     if (rc) return rc;
   }
   if (bsplit && p->n_bnarrow > 0) {
     rc = launch_blu_class<true>(p, p->d_rings_bnarrow, p->n_bnarrow, p->max_r_bnarrow,
-                                p->max_L_bnarrow, S, G, F, maps, epi, stream);
+                                p->max_L_bnarrow, S, G, F, maps, epi, ref, stream);
     if (rc) return rc;
   }
   if (wide) {
     switch (p->r_wide) {
-      case 256: rc = launch_synth_wide<256, 1>(p, S, G, F, maps, epi, stream); break;
-      case 512: rc = launch_synth_wide<256, 2>(p, S, G, F, maps, epi, stream); break;
-      case 1024: rc = launch_synth_wide<256, 4>(p, S, G, F, maps, epi, stream); break;
-      case 2048: rc = launch_synth_wide<256, 8>(p, S, G, F, maps, epi, stream); break;
-      default: rc = launch_synth_wide<512, 8>(p, S, G, F, maps, epi, stream); break;
+      case 256: rc = launch_synth_wide<256, 1>(p, S, G, F, maps, epi, ref, stream); break;
+      case 512: rc = launch_synth_wide<256, 2>(p, S, G, F, maps, epi, ref, stream); break;
+      case 1024: rc = launch_synth_wide<256, 4>(p, S, G, F, maps, epi, ref, stream); break;
+      case 2048: rc = launch_synth_wide<256, 8>(p, S, G, F, maps, epi, ref, stream); break;
+      default: rc = launch_synth_wide<512, 8>(p, S, G, F, maps, epi, ref, stream); break;
     }
     if (rc) return rc;
   }
@@ -1545,22 +1559,22 @@ int launch_ring_synth2_f32(const ga_plan *p, int S, int G, const float *F, float
     const int n = np2;
     // LDS per workgroup: 4 arrays of N padded complex floats (67.6 KB at N =
     // 2048, 135 KB at 4096: one workgroup per CU, so 512 threads there)
-    if (Np <= 256) rc = launch_synth2<256, 1, false>(p, l, n, Np, S, G, F, maps, epi, stream);
-    else if (Np <= 512) rc = launch_synth2<256, 2, false>(p, l, n, Np, S, G, F, maps, epi, stream);
-    else if (Np <= 1024) rc = launch_synth2<256, 4, false>(p, l, n, Np, S, G, F, maps, epi, stream);
-    else if (Np <= 2048) rc = launch_synth2<256, 8, false>(p, l, n, Np, S, G, F, maps, epi, stream);
-    else rc = launch_synth2<512, 8, false>(p, l, n, Np, S, G, F, maps, epi, stream);
+    if (Np <= 256) rc = launch_synth2<256, 1, false>(p, l, n, Np, S, G, F, maps, epi, ref, stream);
+    else if (Np <= 512) rc = launch_synth2<256, 2, false>(p, l, n, Np, S, G, F, maps, epi, ref, stream);
+    else if (Np <= 1024) rc = launch_synth2<256, 4, false>(p, l, n, Np, S, G, F, maps, epi, ref, stream);
+    else if (Np <= 2048) rc = launch_synth2<256, 8, false>(p, l, n, Np, S, G, F, maps, epi, ref, stream);
+    else rc = launch_synth2<512, 8, false>(p, l, n, Np, S, G, F, maps, epi, ref, stream);
     if (rc) return rc;
   }
   if (nb2 > 0) {
     const int32_t *l = lb2;
     const int n = nb2;
     if (Nb <= 2048) {
-      if (rb <= 256) rc = launch_synth2<256, 1, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
-      else if (rb <= 512) rc = launch_synth2<256, 2, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
-      else rc = launch_synth2<256, 4, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+      if (rb <= 256) rc = launch_synth2<256, 1, true>(p, l, n, Nb, S, G, F, maps, epi, ref, stream);
+      else if (rb <= 512) rc = launch_synth2<256, 2, true>(p, l, n, Nb, S, G, F, maps, epi, ref, stream);
+      else rc = launch_synth2<256, 4, true>(p, l, n, Nb, S, G, F, maps, epi, ref, stream);
     } else {
-      rc = launch_synth2<512, 4, true>(p, l, n, Nb, S, G, F, maps, epi, stream);
+      rc = launch_synth2<512, 4, true>(p, l, n, Nb, S, G, F, maps, epi, ref, stream);
     }
   }
   return rc;
@@ -1659,6 +1673,17 @@ int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float 
     clear_error();
   }
   return launch_ring<float>(p, S, G, F, maps, epi, false, s);
+}
+// out = ref - synthesis(F) in one pass (the residual of the analysis
+// iterations); GA_ERR_UNSUPPORTED (nothing launched) when the fused-pass form
+// is not in use or does not cover the plan, and the caller then synthesises
+// and subtracts separately
+int launch_ring_resid_f32(const ga_plan *p, int S, int G, const float *F, float *out,
+                          const float *ref, hipStream_t s) {
+  if (tuning().ring_path == 1) return GA_ERR_UNSUPPORTED;
+  const int rc = launch_ring_synth2_f32(p, S, G, F, out, nullptr, tuning().ring_path != 2, s, ref);
+  if (rc == GA_ERR_UNSUPPORTED) clear_error();
+  return rc;
 }
 int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s) {
   return launch_ring<double>(p, S, S, maps, F, nullptr, true, s);
