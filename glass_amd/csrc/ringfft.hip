@@ -1506,6 +1506,226 @@ int launch_blu_class(const ga_plan *p, const int32_t *list, int nlist, int max_r
                                                    epi, ref, stream);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 analysis (pixels -> ring coefficients), register form.  The mirror of
+// the synthesis: x_{4 j1 + j2} = p_s0 + i p_s1 (thread owns j1 = t + c NT,
+// one 16-byte load per shell), four length-r forward DFTs
+// D_{j2}[k1] = sum_{j1} x_{4 j1 + j2} e^{-2 pi i j1 k1 / r} two at a time on
+// two LDS arrays (power of two: forward DIF passes, results read at the
+// bit-reversed slots; Bluestein: D = conj(c IDFT(conj(x) c)) with the
+// synthesis convolution), then in registers
+//   X_{k1 + r k2} = sum_{j2} (-i)^{j2 k2} e^{-2 pi i j2 k1 / n} D_{j2}[k1].
+// The two real transforms are split with the partner bin n - k, exchanged
+// through LDS in two halves (bins k2 = 2, 3 first, for the outputs of
+// k2 = 0, 1; then the other way), and every m = k (mod n), m <= lmax, gets
+// weight e^{-i m phi0} (d0, d1) stored as one 16-byte pair.
+template <int NT, int C, bool BLU>
+__global__ __launch_bounds__(NT, 4) void ring_anal2_kernel(
+    int lmax, const int32_t *__restrict__ ring_list, int nlist,
+    const int64_t *__restrict__ ring_start, const int32_t *__restrict__ ring_n,
+    const int32_t *__restrict__ ring_pnum, const int32_t *__restrict__ ring_pden,
+    const int64_t *__restrict__ blu_off, const float2 *__restrict__ blu,
+    const float2 *__restrict__ tw, int twlen, int ldx, const float *__restrict__ maps, int S,
+    int nunit, int64_t npix, float weight, cplx<float> *__restrict__ F) {
+  extern __shared__ char smem_raw[];
+  cplx<float> *X = reinterpret_cast<cplx<float> *>(smem_raw);
+  int li, unit;
+  if (!decode_ring_block(nlist, nunit, li, unit)) return;
+  const int ring = ring_list[li];
+  const int t = threadIdx.x;
+  const int s0 = 2 * unit;
+  const bool two = s0 + 1 < S;
+  const int n = ring_n[ring], r = n >> 2;
+  const int pnum = ring_pnum[ring], pden = ring_pden[ring];
+  const int64_t st = ring_start[ring];
+  const float2 *__restrict__ chirp = BLU ? blu + blu_off[r] : blu;
+  const float2 *__restrict__ bspec = chirp + r;
+  const int L = BLU ? blu_len(r) : r;
+  const int lbits = ilog2(r);
+
+  // ---- pixels
+  cplx<float> x[C][4];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int j1 = t + c * NT;
+    float4 a = float4{0.0f, 0.0f, 0.0f, 0.0f}, b = a;
+    if (j1 < r) {
+      a = reinterpret_cast<const float4 *>(maps + int64_t(s0) * npix + st)[j1];
+      if (two) b = reinterpret_cast<const float4 *>(maps + int64_t(s0 + 1) * npix + st)[j1];
+    }
+    x[c][0] = cplx<float>{a.x, b.x};
+    x[c][1] = cplx<float>{a.y, b.y};
+    x[c][2] = cplx<float>{a.z, b.z};
+    x[c][3] = cplx<float>{a.w, b.w};
+  }
+
+  // ---- the four forward DFTs, two at a time; results back into x
+#pragma unroll
+  for (int q0 = 0; q0 < 4; q0 += 2) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int j1 = t + c * NT;
+        if (j1 < r) {
+          cplx<float> v = x[c][q0 + a];
+          if (BLU) {
+            const float2 cc = chirp[j1];
+            v = cmul(cconj(v), cplx<float>{cc.x, cc.y});
+          }
+          X[a * ldx + pad(j1)] = v;
+        }
+      }
+    if (BLU) {
+      for (int e = t; e < (L - r) * 2; e += NT) {
+        const int a = e / (L - r), j = r + e - a * (L - r);
+        X[a * ldx + pad(j)] = cplx<float>{0.0f, 0.0f};
+      }
+    }
+    __syncthreads();
+    if (BLU)
+      bluestein_conv<NT>(X, ldx, 2, L, tw, twlen, bspec);
+    else
+      fused_fft<NT, false, -1>(X, ldx, 2, r, tw, twlen);  // natural -> bit-reversed
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int k1 = t + c * NT;
+        if (k1 < r) {
+          if (BLU) {
+            const float2 cc = chirp[k1];
+            x[c][q0 + a] = cconj(cmul(X[a * ldx + pad(k1)], cplx<float>{cc.x, cc.y}));
+          } else {
+            const int pos = lbits ? int(__brev(unsigned(k1)) >> (32 - lbits)) : 0;
+            x[c][q0 + a] = X[a * ldx + pad(pos)];
+          }
+        }
+      }
+    __syncthreads();
+  }
+
+  // ---- radix-4 combination: bins k1 + r k2 into x[c][k2]
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int k1 = min(t + c * NT, r - 1);
+    const cplx<float> w1 = cconj(expi_q(k1, n, 0.0f));
+    const cplx<float> w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    const cplx<float> y0 = x[c][0], y1 = cmul(x[c][1], w1), y2 = cmul(x[c][2], w2),
+                      y3 = cmul(x[c][3], w3);
+    const cplx<float> s02 = cadd(y0, y2), d02 = csub(y0, y2);
+    const cplx<float> s13 = cadd(y1, y3), d13 = cmuli(csub(y3, y1));  // -i (y1 - y3)
+    x[c][0] = cadd(s02, s13);
+    x[c][1] = cadd(d02, d13);
+    x[c][2] = csub(s02, s13);
+    x[c][3] = csub(d02, d13);
+  }
+
+  // ---- split the two real transforms and unfold, in two halves: the
+  // partner of bin k1 + r k2 is n - k = (r - k1) + r (3 - k2) for k1 > 0,
+  // r ((4 - k2) mod 4) for k1 = 0
+  cplx<float> *Fr = F + int64_t(ring) * (lmax + 1) * S + s0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    // LDS holds the other half: bins k2 = 2 - 2h + a at X[a ldx + pad(k1)]
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k1 = t + c * NT;
+      if (k1 < r) {
+        X[pad(k1)] = x[c][2 - 2 * h];
+        X[ldx + pad(k1)] = x[c][3 - 2 * h];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int k1 = t + c * NT;
+      if (k1 >= r) continue;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int k2 = 2 * h + b;
+        const int k = k1 + r * k2;
+        if (k > lmax) continue;
+        // partner bin (p1, p2): in the other half, or the bin itself (k = 0, n/2)
+        const int p1 = k1 ? r - k1 : 0, p2 = k1 ? 3 - k2 : (4 - k2) & 3;
+        const cplx<float> pv = p2 == k2 ? x[c][k2] : X[(p2 - (2 - 2 * h)) * ldx + pad(p1)];
+        const cplx<float> av = x[c][k2], bv = cconj(pv);
+        const cplx<float> d0 = {0.5f * (av.re + bv.re), 0.5f * (av.im + bv.im)};
+        const cplx<float> d1 = {0.5f * (av.im - bv.im), -0.5f * (av.re - bv.re)};  // (a - b) / (2i)
+        // ring phase e^{-i m phi0}, m phi0 = 2 pi (m pnum mod pden) / pden, stepped by n
+        int q = pnum ? int(int64_t(k) * pnum % pden) : 0;
+        const int dq = pnum ? int(int64_t(n) * pnum % pden) : 0;
+        for (int mm = k; mm <= lmax; mm += n) {
+          const cplx<float> ph = pnum ? cconj(expi_q(q, pden, 0.0f)) : cplx<float>{1.0f, 0.0f};
+          q += dq;
+          if (q >= pden) q -= pden;
+          const cplx<float> v0 = cmul(d0, ph), v1 = cmul(d1, ph);
+          cplx<float> *f = Fr + int64_t(mm) * S;
+          if (two && !(S & 1)) {  // the pair is 16-byte aligned
+            *reinterpret_cast<float4 *>(f) =
+                float4{weight * v0.re, weight * v0.im, weight * v1.re, weight * v1.im};
+          } else {
+            f[0] = cplx<float>{weight * v0.re, weight * v0.im};
+            if (two) f[1] = cplx<float>{weight * v1.re, weight * v1.im};
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NT, int C, bool BLU>
+int launch_anal2(const ga_plan *p, const int32_t *list, int nlist, int maxN, int S,
+                 const float *maps, float *F, hipStream_t stream) {
+  if (nlist == 0) return GA_OK;
+  const int ldx = padded_len(maxN);
+  const size_t lds = size_t(2) * ldx * sizeof(cplx<float>);
+  GA_REQUIRE(lds <= 160 * 1024, GA_ERR_UNSUPPORTED, "ring analysis needs %zu bytes of LDS", lds);
+  const int nunit = (S + 1) / 2;
+  const int nblk = 8 * ((nlist + 7) / 8) * nunit;
+  auto kfn = ring_anal2_kernel<NT, C, BLU>;
+  GA_CHECK_HIP(hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   int(lds)));
+  hipLaunchKernelGGL(kfn, dim3(unsigned(nblk)), dim3(NT), lds, stream, p->lmax, list, nlist,
+                     p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden, p->d_blu_off,
+                     p->d_blu32, p->d_tw32, p->tw_len, ldx, maps, S, nunit, p->npix,
+                     float(4.0 * M_PI / double(p->npix)), reinterpret_cast<cplx<float> *>(F));
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+// both ring classes by the register form; GA_ERR_UNSUPPORTED (before any
+// launch) outside its sizes or alignment
+int launch_ring_anal2_f32(const ga_plan *p, int S, const float *maps, float *F,
+                          hipStream_t stream) {
+  if (reinterpret_cast<uintptr_t>(maps) & 15) return GA_ERR_UNSUPPORTED;
+  if (S > 1 && (reinterpret_cast<uintptr_t>(F) & 15)) return GA_ERR_UNSUPPORTED;
+  const int rp = p->max_r_pow2, rb = p->max_r_blu, Lb = p->max_L_blu;
+  if (p->n_pow2 > 0 && rp > 4096) return GA_ERR_UNSUPPORTED;
+  if (p->n_blu > 0 && (rb > 2048 || Lb > 4096)) return GA_ERR_UNSUPPORTED;
+  int rc = GA_OK;
+  if (p->n_pow2 > 0) {
+    const int32_t *l = p->d_rings_pow2;
+    const int n = p->n_pow2;
+    if (rp <= 256) rc = launch_anal2<256, 1, false>(p, l, n, rp, S, maps, F, stream);
+    else if (rp <= 512) rc = launch_anal2<256, 2, false>(p, l, n, rp, S, maps, F, stream);
+    else if (rp <= 1024) rc = launch_anal2<256, 4, false>(p, l, n, rp, S, maps, F, stream);
+    else if (rp <= 2048) rc = launch_anal2<256, 8, false>(p, l, n, rp, S, maps, F, stream);
+    else rc = launch_anal2<512, 8, false>(p, l, n, rp, S, maps, F, stream);
+    if (rc) return rc;
+  }
+  if (p->n_blu > 0) {
+    const int32_t *l = p->d_rings_blu;
+    const int n = p->n_blu;
+    if (rb <= 256) rc = launch_anal2<256, 1, true>(p, l, n, Lb, S, maps, F, stream);
+    else if (rb <= 512) rc = launch_anal2<256, 2, true>(p, l, n, Lb, S, maps, F, stream);
+    else if (rb <= 1024) rc = launch_anal2<256, 4, true>(p, l, n, Lb, S, maps, F, stream);
+    else rc = launch_anal2<512, 4, true>(p, l, n, Lb, S, maps, F, stream);
+  }
+  return rc;
+}
+
 // both ring classes with the fused-pass kernels (the wide power-of-two rings
 // by the register-fold kernel when wide_rings); GA_ERR_UNSUPPORTED (before any
 // launch) if a class exceeds the instantiated sizes, and the caller then uses
@@ -1689,6 +1909,11 @@ int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F,
   return launch_ring<double>(p, S, S, maps, F, nullptr, true, s);
 }
 int launch_ring_anal_f32(const ga_plan *p, int S, const float *maps, float *F, hipStream_t s) {
+  if (tuning().ring_path != 1) {
+    const int rc = launch_ring_anal2_f32(p, S, maps, F, s);
+    if (rc != GA_ERR_UNSUPPORTED) return rc;
+    clear_error();
+  }
   return launch_ring<float>(p, S, S, maps, F, nullptr, true, s);
 }
 

@@ -246,6 +246,37 @@ def test_analysis_mfma_matches_valu_and_fp64(nside, lmax, S, niter):
         assert np.max(np.abs(out[2][s] - ref)) / scale < 1e-5, s
 
 
+@pytest.mark.parametrize("nside,lmax,S,niter", [(16, 40, 3, 0), (64, 150, 4, 1), (256, 600, 5, 0),
+                                                (512, 1023, 6, 0)])
+def test_ring_analysis_forms_agree(nside, lmax, S, niter):
+    """FP32 batched analysis with the register-form ring stage (default)
+    against the first form (radix-4 LDS stages, GA_TUNE_RING_PATH = 1):
+    max |difference| / RMS of the alm < 2e-6 (power-of-two and Bluestein
+    rings, rings with n <= lmax and > lmax, odd S), and against FP64 < 1e-5."""
+    from glass_amd import _lib
+
+    print(f"ring analysis nside={nside} lmax={lmax} S={S} niter={niter}", flush=True)
+    rng = np.random.default_rng(3 * nside + S)
+    maps = torch.as_tensor(rng.standard_normal((S, 12 * nside * nside))).cuda()
+    m32 = maps.float().contiguous()
+    v = _lib.ctypes.c_int32()
+    _lib.call("ga_get_tuning", _lib.TUNE_RING_PATH, _lib.ctypes.byref(v))
+    out = {}
+    try:
+        for path in (0, 1):
+            _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, path)
+            out[path] = hp.map2alm_batch_f32(m32, lmax, niter).T.cpu().numpy()
+    finally:
+        _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, v.value)
+    for s in range(S):
+        scale = np.sqrt(np.mean(np.abs(out[1][s]) ** 2))
+        assert np.max(np.abs(out[0][s] - out[1][s])) / scale < 2e-6, s
+    for s in (0, S - 1):
+        ref = hp.map2alm(maps[s], lmax=lmax, iter=niter, pol=False).cpu().numpy()
+        scale = np.sqrt(np.mean(np.abs(ref) ** 2))
+        assert np.max(np.abs(out[0][s] - ref)) / scale < 1e-5, s
+
+
 @pytest.mark.parametrize("nside,lmax,S", [(32, 63, 40), (128, 255, 64), (64, 150, 100)])
 def test_wave_pair_legendre_matches_single_wave(nside, lmax, S):
     """The producer / consumer wave-pair form of the split-FP16 Legendre stage
