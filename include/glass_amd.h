@@ -57,10 +57,10 @@ enum ga_tune_key {
                              producer / consumer wave pairs (split FP16 only) */
   GA_TUNE_MFMA_CHUNK = 2, /* FP32 path: quads per accumulation chunk, 0 = none */
   GA_TUNE_MFMA_PATH = 3,  /* 0 = automatic (split FP16), 1 = FP32 MFMA, 2 = split FP16 */
-  GA_TUNE_RING_PATH = 4,  /* FP32 ring synthesis: 0 = automatic (default: fused
-                             radix-2^K passes, the wide power-of-two rings by the
-                             register-fold kernel), 1 = radix-4 LDS stages,
-                             2 = fused passes on every ring */
+  GA_TUNE_RING_PATH = 4,  /* FP32 ring stage: 0 = automatic (default: synthesis
+                             by the register-fold / register-split kernels, analysis
+                             by the register form), 1 = radix-4 LDS stages (both),
+                             2 = synthesis by four-array fused passes on every ring */
   GA_TUNE_SPIN_PATH = 5,  /* FP32 spin synthesis: 0 = automatic (split FP16 MFMA),
                              1 = VALU, 2 = split FP16 MFMA */
   GA_TUNE_ANAL_PATH = 6   /* FP32 analysis Legendre stage: 0 = automatic (split FP16
