@@ -1301,8 +1301,9 @@ int launch_synth_wide(const ga_plan *p, int S, int G, const float *F, float *map
 // thread's k1 = t + c NT, c < C), then the four Bluestein convolutions run A
 // at a time on A LDS arrays of Lmax (A = 2: half the LDS of the four-array
 // form): the remaining split outputs and the finished results
-// wait in registers.  Launched with A = 2.  Same arithmetic, term order and passes as
-// ring_synth2_kernel with BLU.
+// wait in registers.  Launched with A = 2.  Same arithmetic, term order and
+// passes as ring_synth2_kernel with BLU.
+
 // radix-4 split of the bins h[k2] = bin k1 + r k2 of a Bluestein ring and the
 // input chirp: y[q] = (sum_k2 i^(q k2) h[k2]) e^{2 pi i q k1 / n} chirp[k1]
 __device__ __forceinline__ void split_chirp(const cplx<float> (&h)[4], int k1, int n,
