@@ -133,7 +133,7 @@ static void dft_any(double *re, double *im, int64_t n, int sign) {
   while (L < 2 * n - 1) L <<= 1;
   double *ar = calloc((size_t)L, sizeof(double)), *ai = calloc((size_t)L, sizeof(double));
   double *br = calloc((size_t)L, sizeof(double)), *bi = calloc((size_t)L, sizeof(double));
-  double *cr = malloc((size_t)n * sizeof(double)), *ci = malloc((size_t)n * sizeof(double));
+  double *cr = calloc((size_t)n, sizeof(double)), *ci = calloc((size_t)n, sizeof(double));
   /* chirp c_k = exp(sign * i pi k^2 / n), k^2 reduced mod 2n exactly */
   for (int64_t k = 0; k < n; ++k) {
     int64_t q = (int64_t)(((__int128)k * k) % (2 * n));
