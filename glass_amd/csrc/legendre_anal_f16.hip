@@ -38,6 +38,7 @@ using f16::split_pair;
 constexpr int kAWaves = 8;                // waves per workgroup
 constexpr int kAPairs = 64 * kAWaves;     // ring pairs per workgroup = one partial slot
 constexpr int kASlab = 64;                // l rows per slab
+constexpr int kARowPad = 8;               // halves of padding per LDS row of lambda
 constexpr int kAColTop = 13;              // column maxima are scaled below 2^13
 
 __device__ __forceinline__ int anal_shift(float mx) {
@@ -122,21 +123,24 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     __builtin_amdgcn_sched_barrier(0);  // one tile's raw values live at a time
   }
 
-  // LDS: per wave the transposed slab lam[hi|lo][parity][l' (32)][pair (64)]
-  // (16 KB); after the MFMAs of a slab the same memory holds the waves' D
-  // tiles for the fixed-order sum.  Recurrence coefficients (a, a b) go
-  // through a per-wave double buffer of one slab.
-  __shared__ __align__(16) _Float16 lam[kAWaves][2][2][32][64];
-  __shared__ double2 cbuf[kAWaves][2][kASlab];
+  // LDS: per wave the transposed slab lam[hi|lo][parity][l' (32)][pair (64)],
+  // rows padded by 8 halves (16 bytes) so that the 16-byte A reads of 8
+  // consecutive lanes (rows 144 bytes apart) fall in distinct banks (18 KB);
+  // after the MFMAs of a slab the same memory holds the waves' D tiles for
+  // the fixed-order sum.  Recurrence coefficients (a, a b) are the same for
+  // every wave of the workgroup (one m): one double buffer of one slab,
+  // written by wave 0, read after the slab's barriers.
+  __shared__ __align__(16) _Float16 lam[kAWaves][2][2][32][64 + kARowPad];
+  __shared__ double2 cbuf[2][kASlab];
   float *red = reinterpret_cast<float *>(&lam[0][0][0][0][0]);  // [wave][p][e][lane]
 
   const int64_t base = hp_index(lmax, m, m);
   const double2 *__restrict__ co = aab + base;
   const int lastc = lmax - m;
   double2 cpre = co[min(lane, lastc)];
-  cbuf[wave][0][lane] = cpre;
-  __builtin_amdgcn_wave_barrier();
-  _Float16(*lw)[2][32][64] = lam[wave];  // [hi|lo][p][l'][pair]
+  if (wave == 0) cbuf[0][lane] = cpre;
+  __syncthreads();
+  _Float16(*lw)[2][32][64 + kARowPad] = lam[wave];  // [hi|lo][p][l'][pair]
 
   const int nsl = (lmax - m + kASlab) / kASlab;
   // Leading slabs in which every lane of the workgroup either starts scaled
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     double mx = 0.0;
 #pragma unroll 8
     for (int j = 0; j < kASlab; ++j) {
-      const double2 c = cbuf[wave][cur][j];
+      const double2 c = cbuf[cur][j];
       const double pn = fma(c.x * x, p1, -(c.y * p0));
       p0 = p1;
       p1 = pn;
@@ -166,8 +170,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       p0 *= 0x1p-600;
       ++k;
     }
-    cbuf[wave][cur ^ 1][lane] = cpre;
-    __builtin_amdgcn_wave_barrier();
+    if (wave == 0) cbuf[cur ^ 1][lane] = cpre;  // read after this slab's barriers
     if (!__syncthreads_and(kq < 0 || mx < 0x1p-25)) {  // values in this slab: undo it
       p0 = q0;
       p1 = q1;
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     const int l0 = m + kASlab * s;
     const int cur = s & 1;
     cpre = co[min(kASlab * (s + 1) + lane, lastc)];
-    const double2 *__restrict__ cb = cbuf[wave][cur];
+    const double2 *__restrict__ cb = cbuf[cur];
     const bool slow = (l0 + kASlab - 1 > lmax) || wave_any(k < 0);
     // 16 groups of 4 steps: step i -> parity i & 1, row l' = i >> 1
 #pragma unroll 2
@@ -241,8 +244,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       lw[1][1][2 * g][lane] = L1[0];
       lw[1][1][2 * g + 1][lane] = L1[1];
     }
-    cbuf[wave][cur ^ 1][lane] = cpre;
-    __builtin_amdgcn_wave_barrier();
+    if (wave == 0) cbuf[cur ^ 1][lane] = cpre;  // read after this slab's barriers
 
     // MFMAs: A[l'][pair] of k-block kb: lane L reads row L % 32, pairs
     // 16 kb + 8 (L / 32) .. + 7 (16 bytes); one A read serves the NT tiles
