@@ -166,13 +166,17 @@ __global__ void fields_alm_kernel(int nl, int k, int first_shell, int nshell, in
 
 // Device-RNG form, one thread per healpy index walking the shells in order:
 // each normal vector entry z_t[g] is generated once and kept in a sliding
-// register window of k+1 values; the [idx][S] output is staged through LDS in
+// register window of K+1 values (K, the number of correlated neighbours, is a
+// template argument so that the window moves by register renaming rather than
+// by per-entry selects); the weights of the next shell are loaded while the
+// current one is generated; the [idx][S] output is staged through LDS in
 // chunks of 16 shells so that stores are contiguous 128-byte rows.
-constexpr int kWinMax = 8;  // k + 1 <= kWinMax
+constexpr int kWinMax = 8;  // K + 1 <= kWinMax
 constexpr int kFaBlock = 256, kFaChunk = 16;
 
+template <int K>
 __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
-    int nl, int k, int first_shell, int nshell, int64_t nalm, const double *__restrict__ w,
+    int nl, int first_shell, int nshell, int64_t nalm, const double *__restrict__ w,
     uint64_t seed, float2 *__restrict__ out32, int ldo) {
   __shared__ float2 tile[kFaBlock][kFaChunk + 1];
   const int tid = threadIdx.x;
@@ -182,41 +186,56 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
   int l = 0, m = 0;
   if (valid) hp_decode(i, nl - 1, l, m);
   const int64_t g = int64_t(l) * (l + 1) / 2 + m;
-  // window zz[j] = z_{t - k + j} for the current shell t
-  double2 zz[kWinMax];
+  // window zz[j] = z_{t - K + j} for the current shell t
+  double2 zz[K + 1];
 #pragma unroll
-  for (int j = 0; j < kWinMax; ++j) {
-    const int64_t tz = int64_t(first_shell) - k + j;
-    zz[j] = (j <= k && tz >= 0 && valid) ? device_normal_pair(seed, tz, g) : double2{0.0, 0.0};
+  for (int j = 0; j <= K; ++j) {
+    const int64_t tz = int64_t(first_shell) - K + j;
+    zz[j] = (tz >= 0 && valid) ? device_normal_pair(seed, tz, g) : double2{0.0, 0.0};
   }
+  // weights of shell s for this l: wrow[s * wstride + j], j = 0..K
+  const double *wrow = w + int64_t(l) * (K + 1);
+  const int64_t wstride = int64_t(nl) * (K + 1);
+  double wc[K + 1];
+#pragma unroll
+  for (int j = 0; j <= K; ++j) wc[j] = wrow[j];
   const int nrows = int(nalm - i0 < kFaBlock ? nalm - i0 : kFaBlock);
   for (int s0 = 0; s0 < nshell; s0 += kFaChunk) {
     const int cnt = nshell - s0 < kFaChunk ? nshell - s0 : kFaChunk;
     for (int c = 0; c < cnt; ++c) {
       const int s = s0 + c;
       const int64_t t = int64_t(first_shell) + s;
+      const int sn = s + 1 < nshell ? s + 1 : s;
+      double wn[K + 1];
+#pragma unroll
+      for (int j = 0; j <= K; ++j) wn[j] = wrow[sn * wstride + j];
       if (s > 0) {
 #pragma unroll
-        for (int j = 0; j + 1 < kWinMax; ++j)
-          if (j < k) zz[j] = zz[j + 1];
-        double2 zn = valid ? device_normal_pair(seed, t, g) : double2{0.0, 0.0};
-#pragma unroll
-        for (int j = 0; j < kWinMax; ++j)
-          if (j == k) zz[j] = zn;
+        for (int j = 0; j < K; ++j) zz[j] = zz[j + 1];
+        zz[K] = valid ? device_normal_pair(seed, t, g) : double2{0.0, 0.0};
       }
-      const int len = int(t + 1 < k + 1 ? t + 1 : k + 1);
-      const int mis = k + 1 - len;
-      const double *wl = w + (int64_t(s) * nl + l) * (k + 1);
       double re = 0.0, im = 0.0;
+      if (t >= K) {
 #pragma unroll
-      for (int j = 0; j < kWinMax; ++j) {
-        if (j >= mis && j <= k) {
-          re = fma(wl[j], zz[j].x, re);
-          im = fma(wl[j], zz[j].y, im);
+        for (int j = 0; j <= K; ++j) {
+          re = fma(wc[j], zz[j].x, re);
+          im = fma(wc[j], zz[j].y, im);
+        }
+      } else {
+        // rows of the first K shells have t + 1 < K + 1 entries: skip the leading ones
+        const int mis = int(K - t);
+#pragma unroll
+        for (int j = 0; j <= K; ++j) {
+          if (j >= mis) {
+            re = fma(wc[j], zz[j].x, re);
+            im = fma(wc[j], zz[j].y, im);
+          }
         }
       }
       if (m == 0) { re += im; im = 0.0; }
       tile[tid][c] = float2{float(re), float(im)};
+#pragma unroll
+      for (int j = 0; j <= K; ++j) wc[j] = wn[j];
     }
     __syncthreads();
     // cooperative store: element (r, c) of the tile -> out[(i0 + r) ldo + s0 + c]
@@ -226,6 +245,14 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
     }
     __syncthreads();
   }
+}
+
+template <int K>
+void launch_fields_window(int nl, int first_shell, int nshell, int64_t nalm, const double *w,
+                          uint64_t seed, float2 *out32, int ldo, hipStream_t stream) {
+  hipLaunchKernelGGL(fields_alm_window_kernel<K>, dim3(unsigned((nalm + kFaBlock - 1) / kFaBlock)),
+                     dim3(kFaBlock),
+                     0, stream, nl, first_shell, nshell, nalm, w, seed, out32, ldo);
 }
 
 __global__ void multalm_kernel(int64_t n, const double2 *__restrict__ alm,
@@ -319,9 +346,18 @@ extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t
   const int64_t n = nalm * nshell;
   const dim3 grid(nblocks(n, 256)), block(256);
   if (!zring && out_f32 && k + 1 <= kWinMax) {
-    hipLaunchKernelGGL(fields_alm_window_kernel, dim3(nblocks(nalm, kFaBlock)), dim3(kFaBlock),
-                       0, (hipStream_t)stream, nl, k, first_shell, nshell, nalm, w, seed,
-                       reinterpret_cast<float2 *>(out_f32), ldo);
+    float2 *o = reinterpret_cast<float2 *>(out_f32);
+    const hipStream_t st = (hipStream_t)stream;
+    switch (k) {
+      case 0: launch_fields_window<0>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      case 1: launch_fields_window<1>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      case 2: launch_fields_window<2>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      case 3: launch_fields_window<3>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      case 4: launch_fields_window<4>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      case 5: launch_fields_window<5>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      case 6: launch_fields_window<6>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+      default: launch_fields_window<7>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
+    }
     GA_LAUNCH_CHECK();
     return GA_OK;
   }
