@@ -131,20 +131,27 @@ __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int 
   const int64_t unit = sg * per_slab + rem;
   const int l0 = m + kSlab * sl;
   const int c = ct * 32 + (L & 31);
-  const int sh = c < ncols ? col_shift(cmax[c]) : 0;
+  const bool cok = c < ncols;
+  const int cc = cok ? c : 0;
+  // all eight loads are issued before any is used (rows outside [lmin, lmax]
+  // and padding columns read the in-range element (m, cc) and are zeroed)
+  const float *__restrict__ col = alm + hp_index(lmax, m, m) * int64_t(ncols) + cc;
+  float raw[8];
+  bool ok[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int l = l0 + p + 16 * (L >> 5) + 2 * j;
+    ok[j] = cok && l <= lmax && l >= lmin;
+    raw[j] = col[int64_t(ok[j] ? l - m : 0) * ncols];
+  }
+  const int sh = cok ? col_shift(cmax[cc]) : 0;
   if (sg == 0 && p == 0 && L < 32) colexp[c] = sh;
-  const float *__restrict__ col = alm + hp_index(lmax, m, m) * int64_t(ncols) + c;
   uint32_t hv[4], lv[4];
 #pragma unroll
   for (int j = 0; j < 8; j += 2) {
-    float v[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int l = l0 + p + 16 * (L >> 5) + 2 * (j + q);
-      v[q] = (c < ncols && l <= lmax && l >= lmin) ? ldexpf(col[int64_t(l - m) * ncols], sh)
-                                                    : 0.0f;
-    }
-    split_pair(v[0], v[1], hv[j / 2], lv[j / 2]);
+    const float v0 = ok[j] ? ldexpf(raw[j], sh) : 0.0f;
+    const float v1 = ok[j + 1] ? ldexpf(raw[j + 1], sh) : 0.0f;
+    split_pair(v0, v1, hv[j / 2], lv[j / 2]);
   }
   B[unit * 128 + L] = uint4{hv[0], hv[1], hv[2], hv[3]};
   B[unit * 128 + 64 + L] = uint4{lv[0], lv[1], lv[2], lv[3]};
