@@ -1364,7 +1364,7 @@ __device__ __forceinline__ void split_from_lds(const cplx<float> *X, int ldf, in
 }
 
 template <int NT, int C, int A, bool LDSFOLD>
-__global__ __launch_bounds__(NT, 4) void ring_synth_wide_blu_kernel(
+__global__ __launch_bounds__(NT, (NT == 256 && C == 4) ? 3 : 4) void ring_synth_wide_blu_kernel(
     int lmax, const int32_t *__restrict__ ring_list, int nlist,
     const int64_t *__restrict__ ring_start, const int32_t *__restrict__ ring_n,
     const int64_t *__restrict__ blu_off, const float2 *__restrict__ blu,
