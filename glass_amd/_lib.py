@@ -66,6 +66,7 @@ _SIGNATURES = {
     "ga_legendre_synth_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_legendre_prepare_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_legendre_contract_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
+    "ga_legendre_prepare_colmax_batch_f32": [_vp, _i32, _vp, _vp, _i64, _vp, _sz, _vp],
     "ga_ring_synth_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp],
     "ga_map2alm_f64": [_vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
     "ga_map2alm_batch_f32": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
@@ -77,6 +78,8 @@ _SIGNATURES = {
     "ga_iternorm_state_doubles": [_i32, ctypes.POINTER(_i32)],
     "ga_iternorm_batch": [_i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
     "ga_fields_alm": [_i32, _i32, _i32, _i32, _vp, _vp, _i32, _u64, _vp, _i32, _vp, _vp],
+    "ga_fields_colmax_rows": [_i32, ctypes.POINTER(_i64)],
+    "ga_fields_alm_colmax": [_i32, _i32, _i32, _i32, _vp, _u64, _vp, _i32, _vp, _vp],
     "ga_transform_f64": [_i64, _vp, ctypes.POINTER(Epilogue), _vp, _vp],
 }
 

@@ -136,6 +136,16 @@ class LognormalShells:
         self.w = T.empty_like(self.cov)
         self.flag = T.zeros(1, dtype=T.int32, device="cuda")
         self.alm = T.empty((self.nalm, self.S), dtype=T.complex64, device="cuda")
+        # per-block column maxima written by the field kernel, so that the
+        # operand preparation does not read the whole alm batch for them
+        self.colmax_fused = self.k + 1 <= 8
+        self.nparts = 0
+        self.colpart = None
+        if self.colmax_fused:
+            rows = _lib.ctypes.c_int64()
+            _lib.call("ga_fields_colmax_rows", self.nl, _lib.ctypes.byref(rows))
+            self.nparts = rows.value
+            self.colpart = T.empty((self.nparts, 2 * self.S), dtype=T.float32, device="cuda")
         self.plan = _lib.get_plan(self.nside, self.lmax)
         self.F = _lib.workspace(self.plan.workspace_bytes(self.S, 4))
         self.maps = T.empty((self.S, self.npix), dtype=T.float32, device="cuda")
@@ -153,16 +163,26 @@ class LognormalShells:
         _lib.call("ga_iternorm_batch", self.nl, self.k, self.k, self.k, self.S, 1,
                   _lib.ptr(self.cov), _lib.ptr(self.state), _lib.ptr(self.w),
                   _lib.ptr(self.flag), _lib.stream_ptr())
-        _lib.call("ga_fields_alm", self.nl, self.k, 0, self.S, _lib.ptr(self.w), None, 1,
-                  self.seed, _lib.ptr(self.alm), 0, None, _lib.stream_ptr())
+        if self.colmax_fused:
+            _lib.call("ga_fields_alm_colmax", self.nl, self.k, 0, self.S, _lib.ptr(self.w),
+                      self.seed, _lib.ptr(self.alm), 0, _lib.ptr(self.colpart),
+                      _lib.stream_ptr())
+        else:
+            _lib.call("ga_fields_alm", self.nl, self.k, 0, self.S, _lib.ptr(self.w), None, 1,
+                      self.seed, _lib.ptr(self.alm), 0, None, _lib.stream_ptr())
 
     def stage_legendre(self):
         self.stage_operands()
         self.stage_contract()
 
     def stage_operands(self):
-        _lib.call("ga_legendre_prepare_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),
-                  _lib.ptr(self.F), self.F.numel(), _lib.stream_ptr())
+        if self.colmax_fused:
+            _lib.call("ga_legendre_prepare_colmax_batch_f32", self.plan.h, self.S,
+                      _lib.ptr(self.alm), _lib.ptr(self.colpart), self.nparts, _lib.ptr(self.F),
+                      self.F.numel(), _lib.stream_ptr())
+        else:
+            _lib.call("ga_legendre_prepare_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),
+                      _lib.ptr(self.F), self.F.numel(), _lib.stream_ptr())
 
     def stage_contract(self):
         _lib.call("ga_legendre_contract_batch_f32", self.plan.h, self.S, _lib.ptr(self.alm),

@@ -93,6 +93,17 @@ extern "C" int ga_legendre_prepare_batch_f32(const ga_plan *p, int32_t S, const 
   return launch_legendre_prepare_batch(p, S, alm, ws, (hipStream_t)stream);
 }
 
+extern "C" int ga_legendre_prepare_colmax_batch_f32(const ga_plan *p, int32_t S,
+                                                    const float *alm, const float *colpart,
+                                                    int64_t nparts, void *ws, size_t ws_bytes,
+                                                    void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && alm && colpart && nparts >= 1 && ws, GA_ERR_INVALID, "invalid argument");
+  GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
+             ws_bytes, ring_bytes(p, S, 4));
+  return launch_legendre_prepare_batch(p, S, alm, ws, (hipStream_t)stream, colpart, nparts);
+}
+
 extern "C" int ga_legendre_contract_batch_f32(const ga_plan *p, int32_t S, const float *alm,
                                               void *ws, size_t ws_bytes, void *stream) {
   GA_PLAN_OK(p);

@@ -170,15 +170,22 @@ __global__ void fields_alm_kernel(int nl, int k, int first_shell, int nshell, in
 // template argument so that the window moves by register renaming rather than
 // by per-entry selects); the weights of the next shell are loaded while the
 // current one is generated; the [idx][S] output is staged through LDS in
-// chunks of 16 shells so that stores are contiguous 128-byte rows.
+// chunks of 16 shells so that stores are contiguous 128-byte rows.  With
+// colpart, block b also writes the maxima of |re| and |im| of its rows per
+// shell (colpart[b][2 s + 0/1], the float bits ordered as unsigned integers,
+// as the operand preparation orders them), so that the column maxima of the
+// whole batch take one pass over ceil(nalm / 256) rows instead of nalm.
 constexpr int kWinMax = 8;  // K + 1 <= kWinMax
 constexpr int kFaBlock = 256, kFaChunk = 16;
+static_assert(kFaBlock % 64 == 0 && kFaBlock % kFaChunk == 0 && 2 * kFaChunk <= 32,
+              "column-maximum reduction layout");
 
 template <int K>
 __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
     int nl, int first_shell, int nshell, int64_t nalm, const double *__restrict__ w,
-    uint64_t seed, float2 *__restrict__ out32, int ldo) {
+    uint64_t seed, float2 *__restrict__ out32, int ldo, float *__restrict__ colpart) {
   __shared__ float2 tile[kFaBlock][kFaChunk + 1];
+  __shared__ uint32_t red[kFaBlock / 32][32];
   const int tid = threadIdx.x;
   const int64_t i0 = int64_t(blockIdx.x) * kFaBlock;
   const int64_t i = i0 + tid;
@@ -239,9 +246,51 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
     }
     __syncthreads();
     // cooperative store: element (r, c) of the tile -> out[(i0 + r) ldo + s0 + c]
+    uint32_t mre = 0u, mim = 0u;  // maxima of what this thread stored
     for (int e = tid; e < nrows * cnt; e += kFaBlock) {
       const int r = e / cnt, c = e - r * cnt;
-      out32[(i0 + r) * ldo + s0 + c] = tile[r][c];
+      const float2 v = tile[r][c];
+      out32[(i0 + r) * ldo + s0 + c] = v;
+      mre = max(mre, __float_as_uint(fabsf(v.x)));
+      mim = max(mim, __float_as_uint(fabsf(v.y)));
+    }
+    if (colpart && cnt == kFaChunk) {
+      // a full chunk: thread tid stored column tid % 16 only, so lanes
+      // tid ^ 16, ^ 32, ^ 48 of its wave and then the four waves combine
+      const int lane = tid & 63, wave = tid >> 6;
+      mre = max(mre, uint32_t(__shfl_xor(int(mre), 16, 64)));
+      mim = max(mim, uint32_t(__shfl_xor(int(mim), 16, 64)));
+      mre = max(mre, uint32_t(__shfl_xor(int(mre), 32, 64)));
+      mim = max(mim, uint32_t(__shfl_xor(int(mim), 32, 64)));
+      if (lane < kFaChunk) {
+        red[wave][2 * lane] = mre;
+        red[wave][2 * lane + 1] = mim;
+      }
+      __syncthreads();
+      if (tid < 2 * kFaChunk) {
+        uint32_t m2 = red[0][tid];
+#pragma unroll
+        for (int g = 1; g < kFaBlock / 64; ++g) m2 = max(m2, red[g][tid]);
+        colpart[int64_t(blockIdx.x) * (2 * nshell) + 2 * s0 + tid] = __uint_as_float(m2);
+      }
+    } else if (colpart) {
+      // a partial chunk: thread (rg, cc) takes component cc (re / im of
+      // shell s0 + cc / 2) over rows rg, rg + 8, ...
+      const int cc = tid & 31, rg = tid >> 5;
+      uint32_t mx = 0u;
+      if (cc < 2 * cnt)
+        for (int r = rg; r < nrows; r += kFaBlock / 32) {
+          const float2 v = tile[r][cc >> 1];
+          mx = max(mx, __float_as_uint(fabsf((cc & 1) ? v.y : v.x)));
+        }
+      red[rg][cc] = mx;
+      __syncthreads();
+      if (tid < 2 * cnt) {
+        uint32_t m2 = red[0][tid];
+#pragma unroll
+        for (int g = 1; g < kFaBlock / 32; ++g) m2 = max(m2, red[g][tid]);
+        colpart[int64_t(blockIdx.x) * (2 * nshell) + 2 * s0 + tid] = __uint_as_float(m2);
+      }
     }
     __syncthreads();
   }
@@ -249,10 +298,11 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
 
 template <int K>
 void launch_fields_window(int nl, int first_shell, int nshell, int64_t nalm, const double *w,
-                          uint64_t seed, float2 *out32, int ldo, hipStream_t stream) {
+                          uint64_t seed, float2 *out32, int ldo, float *colpart,
+                          hipStream_t stream) {
   hipLaunchKernelGGL(fields_alm_window_kernel<K>, dim3(unsigned((nalm + kFaBlock - 1) / kFaBlock)),
-                     dim3(kFaBlock),
-                     0, stream, nl, first_shell, nshell, nalm, w, seed, out32, ldo);
+                     dim3(kFaBlock), 0, stream, nl, first_shell, nshell, nalm, w, seed, out32, ldo,
+                     colpart);
 }
 
 __global__ void multalm_kernel(int64_t n, const double2 *__restrict__ alm,
@@ -302,6 +352,20 @@ __global__ void transform_f64_kernel(int64_t n, const double *__restrict__ x, in
 }
 
 inline unsigned nblocks(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+void launch_window(int nl, int k, int first_shell, int nshell, int64_t nalm, const double *w,
+                   uint64_t seed, float2 *o, int ldo, float *colpart, hipStream_t st) {
+  switch (k) {
+    case 0: launch_fields_window<0>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    case 1: launch_fields_window<1>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    case 2: launch_fields_window<2>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    case 3: launch_fields_window<3>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    case 4: launch_fields_window<4>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    case 5: launch_fields_window<5>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    case 6: launch_fields_window<6>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+    default: launch_fields_window<7>(nl, first_shell, nshell, nalm, w, seed, o, ldo, colpart, st); break;
+  }
+}
+
 
 }  // namespace
 
@@ -346,18 +410,8 @@ extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t
   const int64_t n = nalm * nshell;
   const dim3 grid(nblocks(n, 256)), block(256);
   if (!zring && out_f32 && k + 1 <= kWinMax) {
-    float2 *o = reinterpret_cast<float2 *>(out_f32);
-    const hipStream_t st = (hipStream_t)stream;
-    switch (k) {
-      case 0: launch_fields_window<0>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      case 1: launch_fields_window<1>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      case 2: launch_fields_window<2>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      case 3: launch_fields_window<3>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      case 4: launch_fields_window<4>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      case 5: launch_fields_window<5>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      case 6: launch_fields_window<6>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-      default: launch_fields_window<7>(nl, first_shell, nshell, nalm, w, seed, o, ldo, st); break;
-    }
+    launch_window(nl, k, first_shell, nshell, nalm, w, seed, reinterpret_cast<float2 *>(out_f32),
+                  ldo, nullptr, (hipStream_t)stream);
     GA_LAUNCH_CHECK();
     return GA_OK;
   }
@@ -371,6 +425,30 @@ extern "C" int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t
                        first_shell, nshell, nalm, w, nullptr, 1, seed,
                        reinterpret_cast<float2 *>(out_f32), ldo,
                        reinterpret_cast<double2 *>(out_f64));
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+extern "C" int ga_fields_colmax_rows(int32_t nl, int64_t *rows) {
+  clear_error();
+  GA_REQUIRE(nl > 0 && rows, GA_ERR_INVALID, "invalid argument to ga_fields_colmax_rows");
+  *rows = (int64_t(nl) * (nl + 1) / 2 + kFaBlock - 1) / kFaBlock;
+  return GA_OK;
+}
+
+extern "C" int ga_fields_alm_colmax(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
+                                    const double *w, uint64_t seed, float *out_f32, int32_t ldo,
+                                    float *colpart, void *stream) {
+  clear_error();
+  GA_REQUIRE(nl > 0 && k >= 0 && first_shell >= 0 && nshell > 0 && w && out_f32 && colpart,
+             GA_ERR_INVALID, "invalid argument to ga_fields_alm_colmax");
+  GA_REQUIRE(k + 1 <= kWinMax, GA_ERR_UNSUPPORTED,
+             "column maxima need k + 1 <= %d neighbours (k = %d)", kWinMax, k);
+  if (ldo == 0) ldo = nshell;
+  GA_REQUIRE(ldo >= nshell, GA_ERR_INVALID, "ldo %d < nshell %d", ldo, nshell);
+  const int64_t nalm = int64_t(nl) * (nl + 1) / 2;
+  launch_window(nl, k, first_shell, nshell, nalm, w, seed, reinterpret_cast<float2 *>(out_f32),
+                ldo, colpart, (hipStream_t)stream);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }

@@ -15,14 +15,18 @@ int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float 
 // exponents: the bws / colexp parts of the FP32 workspace, f32_workspace) and
 // the contraction proper
 // (rows l < lmin of the B operands are zero)
+// (colpart: nparts rows of per-block column maxima standing in for the alm
+// rows in the column-maximum pass)
 int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
-                                int32_t *colexp, hipStream_t s, int lmin = 0);
+                                int32_t *colexp, hipStream_t s, int lmin = 0,
+                                const float *colpart = nullptr, int64_t nparts = 0);
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s);
 // the batched Legendre stage on the path chosen by the tuning knob; ws is the
 // FP32 workspace with F at its start (prepare + contract = synth)
 int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                  hipStream_t s);
+                                  hipStream_t s, const float *colpart = nullptr,
+                                  int64_t nparts = 0);
 int launch_legendre_contract_batch(const ga_plan *p, int S, const float *alm, void *ws,
                                    hipStream_t s);
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,

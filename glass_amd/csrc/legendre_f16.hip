@@ -581,7 +581,8 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
 }  // namespace
 
 int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
-                                int32_t *colexp, hipStream_t stream, int lmin) {
+                                int32_t *colexp, hipStream_t stream, int lmin,
+                                const float *colpart, int64_t nparts) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   uint4 *B = reinterpret_cast<uint4 *>(bws);
@@ -590,14 +591,17 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
   hipLaunchKernelGGL(fill_u32_kernel, dim3(unsigned((ncols + 255) / 256)), dim3(256), 0, stream,
                      cmax, ncols, 0u);
   GA_LAUNCH_CHECK();
-  const int64_t nrows = p->nalm;
+  // the column maxima over the alm rows, or over per-block maxima of them
+  // (same ncols; the maximum of maxima is the maximum)
+  const float *src = colpart ? colpart : alm;
+  const int64_t nrows = colpart ? nparts : p->nalm;
   int nblk = int(std::min<int64_t>((nrows + 255) / 256, 2048));
-  if (ncols % 4 == 0 && ncols <= 1024 && (reinterpret_cast<uintptr_t>(alm) & 15) == 0)
+  if (ncols % 4 == 0 && ncols <= 1024 && (reinterpret_cast<uintptr_t>(src) & 15) == 0)
     hipLaunchKernelGGL(colmax4_kernel, dim3(unsigned(nblk)), dim3(256), 0, stream, nrows, ncols,
-                       reinterpret_cast<const float4 *>(alm), cmax);
+                       reinterpret_cast<const float4 *>(src), cmax);
   else
     hipLaunchKernelGGL(colmax_kernel, dim3(unsigned(nblk)), dim3(256), 0, stream, nrows, ncols,
-                       alm, cmax);
+                       src, cmax);
   GA_LAUNCH_CHECK();
   GA_REQUIRE(p->lmax < 65535, GA_ERR_UNSUPPORTED, "lmax too large for the operand preparation");
   const int units0 = int(nslab_of(p->lmax, 0)) * 2 * nct;
@@ -648,12 +652,13 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
 }
 
 int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, const float *colpart, int64_t nparts) {
   if (tuning().mfma_path == 1) return GA_OK;  // the FP32 path reads the alm directly
   const f32_ws_layout w = f32_workspace(p, S);
   char *base = static_cast<char *>(ws);
   return launch_legendre_prepare_f16(p, S, alm, base + w.b_off,
-                                     reinterpret_cast<int32_t *>(base + w.e_off), stream);
+                                     reinterpret_cast<int32_t *>(base + w.e_off), stream, 0,
+                                     colpart, nparts);
 }
 
 int launch_legendre_contract_batch(const ga_plan *p, int S, const float *alm, void *ws,

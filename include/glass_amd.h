@@ -129,6 +129,14 @@ int ga_legendre_prepare_batch_f32(const ga_plan *plan, int32_t nshell, const flo
                                   void *workspace, size_t ws_bytes, void *stream);
 int ga_legendre_contract_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
                                    void *workspace, size_t ws_bytes, void *stream);
+/* ga_legendre_prepare_batch_f32 with the per-column maxima taken from nparts
+ * rows of per-block maxima (colpart, [nparts][2 nshell] floats, as written by
+ * ga_fields_alm_colmax for the same alm) instead of from the nalm alm rows;
+ * the workspace it leaves is identical.  No counterpart in the reference:
+ * it saves the batched pipeline one full read of the alm. */
+int ga_legendre_prepare_colmax_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
+                                         const float *colpart, int64_t nparts, void *workspace,
+                                         size_t ws_bytes, void *stream);
 int ga_ring_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *F, float *maps,
                             const ga_epilogue *epi, void *stream);
 
@@ -202,6 +210,15 @@ int ga_iternorm_batch(int32_t nl, int32_t kprev, int32_t k, int32_t kmax, int32_
 int ga_fields_alm(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
                   const double *w, const double *zring, int32_t nslot, uint64_t seed,
                   float *out_f32, int32_t ldo, double *out_f64, void *stream);
+/* ga_fields_alm with the device generator into out_f32 (k + 1 <= 8), which
+ * also writes, for each block of 256 consecutive healpy indices, the maxima
+ * of |re| and |im| per shell: colpart[b][2 s + 0 / 1] for b < *rows of
+ * ga_fields_colmax_rows(nl).  No counterpart in the reference (input of
+ * ga_legendre_prepare_colmax_batch_f32). */
+int ga_fields_colmax_rows(int32_t nl, int64_t *rows);
+int ga_fields_alm_colmax(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
+                         const double *w, uint64_t seed, float *out_f32, int32_t ldo,
+                         float *colpart, void *stream);
 
 /* Pointwise transforms on a float64 map (n pixels), out may equal x.
  * glass/grf/_transformations.py. */

@@ -86,6 +86,26 @@ def _check_shells(nside, lmax, S, shells):
     return out
 
 
+@pytest.mark.parametrize("nside,lmax,S,k", [(128, 300, 20, 3), (64, 150, 5, 0), (128, 255, 37, 7)])
+def test_fused_column_maxima_give_identical_maps(nside, lmax, S, k):
+    """The field kernel's per-block column maxima (ga_fields_alm_colmax) reduce
+    to the column maxima of the alm batch, and the operand preparation from
+    them leaves the same maps as the preparation that reads the alm, bit for
+    bit (37 shells: a partial last chunk of 5; lmax 255: a partial last block)."""
+    gls = shell_spectra(S, lmax, amp=lambda i: 1.0 + i / S,
+                        rho=(0.6, 0.3, 0.1, 0.05, 0.02, 0.01, 0.005))
+    eng = LognormalShells(gls, nside, ncorr=k, seed=3, lmax=lmax)
+    assert eng.colmax_fused
+    eng.run()
+    fused = eng.maps.clone()
+    a = torch.view_as_real(eng.alm).reshape(eng.nalm, 2 * S).abs().amax(dim=0)
+    assert eng.colpart.shape == (-(-eng.nalm // 256), 2 * S)
+    assert torch.equal(eng.colpart.amax(dim=0), a)
+    eng.colmax_fused = False
+    eng.run()
+    assert torch.equal(eng.maps, fused)
+
+
 def test_lognormal_shells_with_shear_small():
     for e_map, e_q, e_u in _check_shells(64, 127, 8, [0, 3, 7]):
         assert e_map < 1e-5 and e_q < 1e-5 and e_u < 1e-5
