@@ -94,6 +94,16 @@ struct ga_plan {
   // lambda_mm = c_m sin^m theta
   double2 *d_ab = nullptr;
   double2 *d_aab = nullptr;  // (a, a b) per (l, m): the same recurrence, one FMA deep
+  // The split-FP16 synthesis runs the recurrence on y_l = lambda_l / c_l with
+  // c_m = c_{m+1} = 1, c_l = a_l b_l c_{l-2}, which makes the lambda_{l-2}
+  // coefficient 1: y_l = (gam_l x) y_{l-1} - y_{l-2}, gam_l = a_l c_{l-1} / c_l
+  // (gam at l = m is 0: one step from (-seed, 0) yields y_m = seed); its B
+  // operand carries c_l a_lm (d_csc).  lam_exp16: the power of two the
+  // recurrence values are carried with, the largest <= 8 that keeps
+  // max_l sqrt((2l+1)/4pi) / c_l times it below 2^15.5 (FP16 range).
+  double *d_gam = nullptr;
+  double *d_csc = nullptr;
+  int lam_exp16 = 8;
   double *d_cm = nullptr;
   // FP16 MFMA stage: l-slabs of kSlab rows per m; slab_off[m] = index of the
   // first slab of m (lmax + 2 entries, slab_off[lmax + 1] = total)

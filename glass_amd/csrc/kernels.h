@@ -16,10 +16,12 @@ int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float 
 // the contraction proper
 // (rows l < lmin of the B operands are zero)
 // (colpart: nparts rows of per-block column maxima standing in for the alm
-// rows in the column-maximum pass)
+// rows in the column-maximum pass; csc: per-(l, m) factors applied to the
+// rows, the c_l of the scalar synthesis's normalised recurrence, NULL for none)
 int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
                                 int32_t *colexp, hipStream_t s, int lmin = 0,
-                                const float *colpart = nullptr, int64_t nparts = 0);
+                                const float *colpart = nullptr, int64_t nparts = 0,
+                                const double *csc = nullptr);
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s);
 // the batched Legendre stage on the path chosen by the tuning knob; ws is the

@@ -11,11 +11,21 @@
 // 16 terms instead of once per 2 (FP32 MFMA): the split path is both faster
 // (FP16 MFMA has 16x the FP32 rate) and no less accurate.
 //
-// Ranges: the normalised lambda_lm is bounded by sqrt((2l+1)/4pi) < 2^6 for
-// l < 16384; the recurrence carries lambda 2^8 so that |lambda 2^8| < 2^14
-// fits FP16 and small values keep their low part in the normal range.  The a_lm
-// of each real column are scaled by a power of two that puts the column's
-// maximum in [2^12, 2^13).  The accumulator is unscaled in the epilogue.
+// The recurrence runs on y_l = lambda_l / c_l (c_m = c_{m+1} = 1,
+// c_l = a_l b_l c_{l-2}), where the lambda_{l-2} coefficient is 1:
+// y_l = (gam_l x) y_{l-1} - y_{l-2}, one FP64 multiply and one FMA per step
+// instead of two multiplies and an FMA; the B operand carries c_l a_lm, so
+// sum_l y_l (c_l a_lm) = sum_l lambda_l a_lm.  c_l stays within [0.14, 1.13]
+// for lmax <= 8192.
+//
+// Ranges: the normalised lambda_lm is bounded by sqrt((2l+1)/4pi), so
+// |y_l| <= max_l sqrt((2l+1)/4pi) / c_l (151 at lmax 4096, 255 at 8192); the
+// recurrence carries y 2^e with the largest e <= 8 that keeps this bound
+// below 2^15.5 (e = 8 up to lmax 4096), so that it fits FP16 and small values
+// keep their low part in the normal range.  The a_lm of each real column are
+// scaled by a power of two that puts the column's maximum in [2^12, 2^13)
+// (times c_l <= 1.13 after the c scaling).  The accumulator is unscaled in the
+// epilogue.
 //
 // Work decomposition as in the FP32 stage: a workgroup = 4 waves = 256 ring
 // pairs of one m and NT 32-column tiles; a lane owns one ring pair and runs the
@@ -36,7 +46,6 @@ namespace ga {
 namespace {
 
 using f16::f32x16;
-using f16::kLamExp;
 using f16::split_pair;
 
 constexpr int kColTop = 13;  // column maxima are scaled below 2^13
@@ -114,10 +123,13 @@ __device__ __forceinline__ int col_shift(uint32_t bits) {
 // L % 32, as 8 high halves (first 64 x 16 bytes) and 8 low halves (next).
 // grid: (ceil(units of m = 0 / 4), lmax + 1): blockIdx.y = m, 4 units per block.
 // Rows l < lmin are stored as zero (the spin-s stages: no modes below l = s).
+// With csc, row l of m is multiplied by csc[(m, l)] (in FP64, rounded once)
+// before the scaling: the c_l of the normalised synthesis recurrence.
 __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int ncols, int nct,
                                                         const int64_t *__restrict__ slab_off,
                                                         const float *__restrict__ alm,
                                                         const uint32_t *__restrict__ cmax,
+                                                        const double *__restrict__ csc,
                                                         uint4 *__restrict__ B,
                                                         int32_t *__restrict__ colexp) {
   const int m = blockIdx.y;
@@ -135,7 +147,8 @@ __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int 
   const int cc = cok ? c : 0;
   // all eight loads are issued before any is used (rows outside [lmin, lmax]
   // and padding columns read the in-range element (m, cc) and are zeroed)
-  const float *__restrict__ col = alm + hp_index(lmax, m, m) * int64_t(ncols) + cc;
+  const int64_t mbase = hp_index(lmax, m, m);
+  const float *__restrict__ col = alm + mbase * int64_t(ncols) + cc;
   float raw[8];
   bool ok[8];
 #pragma unroll
@@ -143,6 +156,13 @@ __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int 
     const int l = l0 + p + 16 * (L >> 5) + 2 * j;
     ok[j] = cok && l <= lmax && l >= lmin;
     raw[j] = col[int64_t(ok[j] ? l - m : 0) * ncols];
+  }
+  if (csc) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int l = l0 + p + 16 * (L >> 5) + 2 * j;
+      raw[j] = float(double(raw[j]) * csc[mbase + (ok[j] ? l - m : 0)]);
+    }
   }
   const int sh = cok ? col_shift(cmax[cc]) : 0;
   if (sg == 0 && p == 0 && L < 32) colexp[c] = sh;
@@ -157,11 +177,13 @@ __global__ __launch_bounds__(256) void swizzle_b_kernel(int lmax, int lmin, int 
   B[unit * 128 + 64 + L] = uint4{lv[0], lv[1], lv[2], lv[3]};
 }
 
-// One slab (32 rows from l0) of a lane's recurrence: lambda 2^8 for its ring
-// pair, split and packed as FP16 pairs (ph / pl [parity][group], group g =
-// rows k = 2g, 2g + 1 of each parity), with the scaled-seed bookkeeping of
-// the turn-on region.  cb: the slab's (a, a b) coefficient pairs in LDS.
-__device__ __forceinline__ void slab_values(const double2 *__restrict__ cb, double x, int l0,
+// One slab (32 rows from l0) of a lane's recurrence: y_l = lambda_l / c_l
+// times 2^lam_exp for its ring pair (y_l = (gam_l x) y_{l-1} - y_{l-2}; the
+// B operand carries c_l a_lm), split and packed as FP16 pairs (ph / pl
+// [parity][group], group g = rows k = 2g, 2g + 1 of each parity), with the
+// scaled-seed bookkeeping of the turn-on region.  cb: the slab's gam_l in LDS
+// (16-byte aligned).
+__device__ __forceinline__ void slab_values(const double *__restrict__ cb, double x, int l0,
                                             int lmax, double &p0, double &p1, int &k,
                                             uint32_t (&ph)[2][8], uint32_t (&pl)[2][8]) {
   const bool slow = (l0 + kSlab - 1 > lmax) || wave_any(k < 0);
@@ -170,27 +192,29 @@ __device__ __forceinline__ void slab_values(const double2 *__restrict__ cb, doub
     // coefficients of group g + 1 are read from LDS while group g runs; the
     // scheduling barrier keeps later groups' reads from being hoisted
     // further (each would hold 8 VGPRs for the rest of the slab).
-    double2 cc[4], cn[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) cc[j] = cb[j];
+    const double2 *__restrict__ cb2 = reinterpret_cast<const double2 *>(cb);
+    double2 cc[2], cn[2];
+    cc[0] = cb2[0];
+    cc[1] = cb2[1];
 #pragma unroll
     for (int g = 0; g < 8; ++g) {
       if (g < 7) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cn[j] = cb[4 * (g + 1) + j];
+        cn[0] = cb2[2 * (g + 1)];
+        cn[1] = cb2[2 * (g + 1) + 1];
       }
+      const double gm[4] = {cc[0].x, cc[0].y, cc[1].x, cc[1].y};
       float v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double pn = fma(cc[j].x * x, p1, -(cc[j].y * p0));
+        const double pn = fma(gm[j] * x, p1, -p0);
         p0 = p1;
         p1 = pn;
         v[j] = float(pn);
       }
       split_pair(v[0], v[2], ph[0][g], pl[0][g]);
       split_pair(v[1], v[3], ph[1][g], pl[1][g]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) cc[j] = cn[j];
+      cc[0] = cn[0];
+      cc[1] = cn[1];
       __builtin_amdgcn_sched_barrier(0);
     }
   } else {
@@ -202,8 +226,8 @@ __device__ __forceinline__ void slab_values(const double2 *__restrict__ cb, doub
       if (lg <= lmax) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const double2 c = cb[4 * g + j];
-          const double pn = fma(c.x * x, p1, -(c.y * p0));
+          const double c = cb[4 * g + j];
+          const double pn = fma(c * x, p1, -p0);
           p0 = p1;
           p1 = pn;
           v[j] = p1;
@@ -240,16 +264,16 @@ __device__ __forceinline__ void slab_values(const double2 *__restrict__ cb, doub
 
 // The dead leading slabs of a wave (see the synthesis kernel); returns the
 // first slab to run in full.  A slab is dead when every lane either starts it
-// scaled (k < 0: all its values are below 2^-300) or keeps |lambda 2^8| below
-// 2^-25 throughout, where both FP16 halves of the split are exactly 0 (the
+// scaled (k < 0: all its values are below 2^-300) or keeps its carried value
+// (y 2^lam_exp) below 2^-25 in magnitude throughout, where both FP16 halves of the split are exactly 0 (the
 // high half rounds toward zero below 2^-24, the low half to nearest): the
 // slab's A operands would be all zero and its MFMAs would add nothing.
 // cb[2][kSlab] is the wave's coefficient double buffer (slab s in
 // cb[s & 1]); cpre carries the prefetched pair.
-__device__ __forceinline__ int skip_dead_slabs(double2 (*cb)[kSlab],
-                                               const double2 *__restrict__ co, int lastc,
+__device__ __forceinline__ int skip_dead_slabs(double (*cb)[kSlab],
+                                               const double *__restrict__ co, int lastc,
                                                int lane, double x, int m, int lmax, int nsl,
-                                               double &p0, double &p1, int &k, double2 &cpre) {
+                                               double &p0, double &p1, int &k, double &cpre) {
   constexpr double kTiny = 0x1p-25;
   int s = 0;
   while (s < nsl && m + kSlab * (s + 1) - 1 <= lmax && wave_all(k < 0 || fabs(p1) < kTiny)) {
@@ -260,8 +284,8 @@ __device__ __forceinline__ int skip_dead_slabs(double2 (*cb)[kSlab],
     double mx = 0.0;
 #pragma unroll 8
     for (int j = 0; j < kSlab; ++j) {
-      const double2 c = cb[cur][j];
-      const double pn = fma(c.x * x, p1, -(c.y * p0));
+      const double c = cb[cur][j];
+      const double pn = fma(c * x, p1, -p0);
       p0 = p1;
       p1 = pn;
       mx = fmax(mx, fabs(pn));
@@ -297,7 +321,7 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
                                            int rpc, int lane, const int32_t *__restrict__ ring_pnum,
                                            const int32_t *__restrict__ ring_pden,
                                            const int32_t *__restrict__ colexp,
-                                           float *__restrict__ F) {
+                                           float *__restrict__ F, int lam_exp) {
   const int half = lane >> 5, col_in = lane & 31;
   float phc, phs;
   {
@@ -311,7 +335,7 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
 #pragma unroll
   for (int ct = 0; ct < NT; ++ct) {
     const int col = colbase + ct * 32 + col_in;
-    fac[ct] = col < ncols ? ldexpf(1.0f, -(kLamExp + colexp[col])) : 0.0f;
+    fac[ct] = col < ncols ? ldexpf(1.0f, -(lam_exp + colexp[col])) : 0.0f;
   }
   const int ngrp = (ncols + 7) / 8;
   const bool is_im = (col_in & 1) != 0;
@@ -347,10 +371,11 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
 template <int NT, int MINW>
 __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
     int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
-    const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
+    const double *__restrict__ z, const double *__restrict__ sth, const double *__restrict__ gam,
     const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
     const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
-    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F) {
+    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F,
+    int lam_exp) {
   // XCD-aware decode: block b runs on XCD b % 8; XCD x takes m = x, x + 8, ...
   // (largest l-range first), all (ring group, column group) tiles of one m
   // back to back so they share that m's B slabs in the XCD's L2.
@@ -366,15 +391,15 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   const int rp = rp0 + lane;
   const int rpc = rp < npairs ? rp : npairs - 1;
   const double x = z[rpc];
-  // the recurrence table holds (a, a b): lambda_l = (a x) lambda_{l-1} - (a b) lambda_{l-2},
-  // so a step has one FMA on its dependency chain (a x and (a b) lambda_{l-2}
-  // are ready a step early)
-  double p0, p1 = 0.0;  // lambda_{l-2}, lambda_{l-1} (times 2^8); the table entry at
-  int k;                // l = m is (1, -1), so one step from (seed, 0) yields lambda_mm
-  legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
+  // the normalised recurrence y_l = (gam_l x) y_{l-1} - y_{l-2}: one multiply
+  // and one FMA per step, the FMA alone on the dependency chain
+  double p0, p1 = 0.0;  // y_{l-2}, y_{l-1} (times 2^lam_exp); gam at l = m is 0,
+  int k;                // so one step from (-seed, 0) yields y_m = lambda_mm
+  legendre_seed(ldexp(cm[m], lam_exp), sth[rpc], m, p0, k);
+  p0 = -p0;
 
   const int64_t base = hp_index(lmax, m, m);
-  const double2 *__restrict__ co = ab + base;
+  const double *__restrict__ co = gam + base;
   const int64_t s0 = slab_off[m];
   const int nsl = int(nslab_of(lmax, m));
   const int half = lane >> 5, col_in = lane & 31;
@@ -397,9 +422,9 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   // one pair each) while the current slab runs, then written to the other
   // buffer; every step reads its pair with a broadcast LDS load.  Only this
   // wave touches its buffers, so no workgroup barrier is needed.
-  __shared__ double2 cbuf[4][2][kSlab];
+  __shared__ __align__(16) double cbuf[4][2][kSlab];
   const int lastc = lmax - m;  // last valid row index of this m
-  double2 cpre = co[min(lane & (kSlab - 1), lastc)];
+  double cpre = co[min(lane & (kSlab - 1), lastc)];
   if (lane < kSlab) cbuf[wave][0][lane] = cpre;
   __builtin_amdgcn_wave_barrier();
 
@@ -427,7 +452,7 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
       }
     // next slab's coefficients (clamped past the end; never used there)
     cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
-    const double2 *__restrict__ cb = cbuf[wave][cur];
+    const double *__restrict__ cb = cbuf[wave][cur];
 
     uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
     slab_values(cb, x, l0, lmax, p0, p1, k, ph, pl);
@@ -440,7 +465,7 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   }
 
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
-                 colexp, F);
+                 colexp, F, lam_exp);
 }
 
 // Producer / consumer form.  In the single-wave form every wave runs the
@@ -455,10 +480,11 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
 // registers, so their loads have two slab periods to arrive.
 __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
     int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
-    const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
+    const double *__restrict__ z, const double *__restrict__ sth, const double *__restrict__ gam,
     const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
     const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
-    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F) {
+    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F,
+    int lam_exp) {
   constexpr int NT = 2;
   // XCD-aware decode as in the single-wave form (ncg counts 64-column groups)
   const int b = blockIdx.x;
@@ -484,14 +510,15 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
   __shared__ int aflag[2][4];
 
   if (producer) {
-    __shared__ double2 cbuf[4][2][kSlab];
+    __shared__ __align__(16) double cbuf[4][2][kSlab];
     const double x = z[rpc];
     double p0, p1 = 0.0;
     int k;
-    legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
-    const double2 *__restrict__ co = ab + base;
+    legendre_seed(ldexp(cm[m], lam_exp), sth[rpc], m, p0, k);
+    p0 = -p0;
+    const double *__restrict__ co = gam + base;
     const int lastc = lmax - m;
-    double2 cpre = co[min(lane & (kSlab - 1), lastc)];
+    double cpre = co[min(lane & (kSlab - 1), lastc)];
     if (lane < kSlab) cbuf[pr][0][lane] = cpre;
     __builtin_amdgcn_wave_barrier();
     for (int s = 0; s < nsl; ++s) {
@@ -575,14 +602,14 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
   }
   if (!active) return;
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
-                 colexp, F);
+                 colexp, F, lam_exp);
 }
 
 }  // namespace
 
 int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void *bws,
                                 int32_t *colexp, hipStream_t stream, int lmin,
-                                const float *colpart, int64_t nparts) {
+                                const float *colpart, int64_t nparts, const double *csc) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   uint4 *B = reinterpret_cast<uint4 *>(bws);
@@ -607,7 +634,7 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
   const int units0 = int(nslab_of(p->lmax, 0)) * 2 * nct;
   hipLaunchKernelGGL(swizzle_b_kernel, dim3(unsigned((units0 + 3) / 4), unsigned(p->lmax + 1)),
                      dim3(256), 0, stream, p->lmax, lmin, ncols, nct, p->d_slab_off, alm, cmax,
-                     B, colexp);
+                     csc, B, colexp);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
@@ -629,8 +656,8 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
     GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
     hipLaunchKernelGGL(legendre_synth_f16_pc_kernel, dim3(unsigned(nblk)), dim3(512), 0, stream,
                        p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, p->d_z, p->d_sth,
-                       p->d_aab, p->d_cm, p->d_ring_pnum, p->d_ring_pden, p->d_slab_off, B, colexp,
-                       F);
+                       p->d_gam, p->d_cm, p->d_ring_pnum, p->d_ring_pden, p->d_slab_off, B, colexp,
+                       F, p->lam_exp16);
     GA_LAUNCH_CHECK();
     return GA_OK;
   }
@@ -642,8 +669,8 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
 #define GA_F16_LAUNCH(NT_, MW_)                                                                \
   hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_>), dim3(unsigned(nblk2)), dim3(256), \
                      0, stream, p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, p->d_z,   \
-                     p->d_sth, p->d_aab, p->d_cm, p->d_ring_pnum, p->d_ring_pden,               \
-                     p->d_slab_off, B, colexp, F)
+                     p->d_sth, p->d_gam, p->d_cm, p->d_ring_pnum, p->d_ring_pden,               \
+                     p->d_slab_off, B, colexp, F, p->lam_exp16)
   if (nt == 1) GA_F16_LAUNCH(1, 2);
   else GA_F16_LAUNCH(2, 2);
 #undef GA_F16_LAUNCH
@@ -658,7 +685,7 @@ int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, voi
   char *base = static_cast<char *>(ws);
   return launch_legendre_prepare_f16(p, S, alm, base + w.b_off,
                                      reinterpret_cast<int32_t *>(base + w.e_off), stream, 0,
-                                     colpart, nparts);
+                                     colpart, nparts, p->d_csc);
 }
 
 int launch_legendre_contract_batch(const ga_plan *p, int S, const float *alm, void *ws,

@@ -6,6 +6,7 @@
 // phi0 = pi / (4 i); equatorial rings nside <= i <= 3 nside hold 4 nside pixels
 // at z = 4/3 - 2 i / (3 nside) with phi0 = pi / (4 nside) when i + nside is even
 // and 0 otherwise; the south cap mirrors the north.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -284,6 +285,32 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
       }
     }
     if ((rc = upload(&p->d_ab, ab.data(), ab.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
+    // normalised form for the split-FP16 synthesis: y_l = lambda_l / c_l
+    {
+      std::vector<double> gam(size_t(p->nalm + 4), 0.0), csc(size_t(p->nalm + 4), 1.0);
+      double bound = 0.0;
+      for (int64_t m = 0; m <= lmax; ++m) {
+        const int64_t b0 = hp_index(lmax, m, m);
+        double cprev2 = 1.0, cprev = 1.0;  // c_{l-2}, c_{l-1}
+        gam[size_t(b0)] = 0.0;
+        csc[size_t(b0)] = 1.0;
+        bound = std::max(bound, std::sqrt((2.0 * double(m) + 1.0) / (4.0 * M_PI)));
+        for (int64_t l = m + 1; l <= lmax; ++l) {
+          const double2 v = ab[size_t(b0 + (l - m))];
+          const double c = l == m + 1 ? 1.0 : v.x * v.y * cprev2;
+          gam[size_t(b0 + (l - m))] = v.x * cprev / c;
+          csc[size_t(b0 + (l - m))] = c;
+          bound = std::max(bound, std::sqrt((2.0 * double(l) + 1.0) / (4.0 * M_PI)) / c);
+          cprev2 = cprev;
+          cprev = c;
+        }
+      }
+      int e = 8;
+      while (e > 0 && std::ldexp(bound, e) >= 0x1p15 * M_SQRT2) --e;
+      p->lam_exp16 = e;
+      if ((rc = upload(&p->d_gam, gam.data(), gam.size() * 8, &p->table_bytes))) return fail(rc);
+      if ((rc = upload(&p->d_csc, csc.data(), csc.size() * 8, &p->table_bytes))) return fail(rc);
+    }
     // the same recurrence as (a, a b): lambda_l = (a x) lambda_{l-1} - (a b) lambda_{l-2},
     // one FMA on the dependency chain per step
     for (double2 &v : ab) v.y *= v.x;
@@ -434,7 +461,7 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
 extern "C" int ga_plan_destroy(ga_plan *p) {
   if (!p) return GA_OK;
   void *ptrs[] = {p->d_z, p->d_sth, p->d_ring_start, p->d_ring_n, p->d_ring_pnum, p->d_ring_pden,
-                  p->d_ab, p->d_aab, p->d_cm, p->d_slab_off, p->d_spin[0], p->d_spin[1], p->d_spin[2],
+                  p->d_ab, p->d_aab, p->d_gam, p->d_csc, p->d_cm, p->d_slab_off, p->d_spin[0], p->d_spin[1], p->d_spin[2],
                   p->d_spin_seed[0], p->d_spin_seed[1], p->d_spin_seed[2], p->d_blu_off,
                   p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32, p->d_rings_pow2,
                   p->d_rings_blu, p->d_rings_wide, p->d_rings_narrow, p->d_rings_bwide,

@@ -66,6 +66,27 @@ def test_alm2map_fp32_mfma(nside, lmax, S):
         assert e < 1e-5, (s, e)
 
 
+def test_alm2map_fp32_mfma_lmax8192():
+    """lmax 8192 (the largest serving size): the normalised recurrence of the
+    split-FP16 synthesis, y_l = lambda_l / c_l, reaches 255 sqrt-normalised
+    units, so the plan carries it times 2^7 instead of 2^8 to stay in FP16
+    range; checked against the FP64 path (itself pinned to the oracle).
+    Tolerance: RMS error <= 1e-6 of the map RMS, max <= 2e-5 (FP32
+    accumulation over l-sums of 8192 terms: measured 8.3e-7 RMS and
+    1.0e-5-1.15e-5 max, with the plain recurrence as with this one)."""
+    nside, lmax, S = 64, 8192, 2
+    print(f"alm2map fp32 nside={nside} lmax={lmax} S={S}", flush=True)
+    rng = np.random.default_rng(8192)
+    alms = [rand_alm(lmax, rng, lambda l: 1e-4 / (l + 1.0) ** 1.2) for _ in range(S)]
+    got = hp.alm2map(alms, nside, pol=False, precision="fp32")
+    for s in range(S):
+        ref = np.asarray(hp.alm2map(alms[s], nside, pol=False, precision="fp64"), dtype=np.float64)
+        g = np.asarray(got[s], dtype=np.float64)
+        rms = float(np.sqrt(np.mean((g - ref) ** 2)) / np.sqrt(np.mean(ref**2)))
+        e = relerr(g, ref)
+        assert rms < 1e-6 and e < 2e-5, (s, rms, e)
+
+
 @pytest.mark.parametrize("nside,lmax,niter", [(4, 8, 0), (8, 16, 0), (8, 12, 3), (16, 40, 3),
                                               (32, 95, 1)])
 def test_map2alm_fp64(nside, lmax, niter):
