@@ -32,7 +32,6 @@ namespace {
 
 using f16::f32x16;
 using f16::f16x8;
-using f16::kLamExp;
 using f16::split_pair;
 
 constexpr int kAWaves = 8;                // waves per workgroup
@@ -51,9 +50,9 @@ __device__ __forceinline__ int anal_shift(float mx) {
 template <int NT>
 __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     int lmax, int npairs, int nrings, int ncols, int nrg, int ncg, const double *__restrict__ z,
-    const double *__restrict__ sth, const double2 *__restrict__ aab,
-    const double *__restrict__ cm, const float *__restrict__ F, float *__restrict__ partial,
-    int64_t nalm) {
+    const double *__restrict__ sth, const double *__restrict__ gam,
+    const double *__restrict__ csc, const double *__restrict__ cm, const float *__restrict__ F,
+    float *__restrict__ partial, int64_t nalm, int lam_exp) {
   // XCD-aware decode (as in the synthesis stages): XCD x takes m = x, x + 8, ...
   const int b = blockIdx.x;
   const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
@@ -70,7 +69,11 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
   const double x = z[rpc];
   double p0, p1 = 0.0;
   int k;
-  legendre_seed(cm[m] * 0x1p8, sth[rpc], m, p0, k);
+  // the normalised recurrence of the synthesis, y_l = lambda_l / c_l:
+  // y_l = (gam_l x) y_{l-1} - y_{l-2}, started from (-seed, 0); the partial
+  // sums of y are multiplied by c_l when stored
+  legendre_seed(ldexp(cm[m], lam_exp), sth[rpc], m, p0, k);
+  p0 = -p0;
   if (!valid) {  // a missing ring pair contributes zeros
     p0 = 0.0;
     k = 0;
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       }
     mx = fmaxf(mx, __shfl_xor(mx, 32, 64));  // lanes L and L ^ 32 hold the same column
     const int sh = anal_shift(mx);
-    fac[ct] = colok ? ldexpf(1.0f, -(kLamExp + sh)) : 0.0f;
+    fac[ct] = colok ? ldexpf(1.0f, -(lam_exp + sh)) : 0.0f;
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
       uint32_t h[2][4], l[2][4];
@@ -127,24 +130,24 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
   // rows padded by 8 halves (16 bytes) so that the 16-byte A reads of 8
   // consecutive lanes (rows 144 bytes apart) fall in distinct banks (18 KB);
   // after the MFMAs of a slab the same memory holds the waves' D tiles for
-  // the fixed-order sum.  Recurrence coefficients (a, a b) are the same for
+  // the fixed-order sum.  Recurrence coefficients gam_l are the same for
   // every wave of the workgroup (one m): one double buffer of one slab,
   // written by wave 0, read after the slab's barriers.
   __shared__ __align__(16) _Float16 lam[kAWaves][2][2][32][64 + kARowPad];
-  __shared__ double2 cbuf[2][kASlab];
+  __shared__ double cbuf[2][kASlab];
   float *red = reinterpret_cast<float *>(&lam[0][0][0][0][0]);  // [wave][p][e][lane]
 
   const int64_t base = hp_index(lmax, m, m);
-  const double2 *__restrict__ co = aab + base;
+  const double *__restrict__ co = gam + base;
   const int lastc = lmax - m;
-  double2 cpre = co[min(lane, lastc)];
+  double cpre = co[min(lane, lastc)];
   if (wave == 0) cbuf[0][lane] = cpre;
   __syncthreads();
   _Float16(*lw)[2][32][64 + kARowPad] = lam[wave];  // [hi|lo][p][l'][pair]
 
   const int nsl = (lmax - m + kASlab) / kASlab;
   // Leading slabs in which every lane of the workgroup either starts scaled
-  // (k < 0: values below 2^-300) or keeps |lambda 2^8| below 2^-25, where both
+  // (k < 0: values below 2^-300) or keeps |y 2^lam_exp| below 2^-25, where both
   // FP16 halves of the split are exactly 0, give partial sums of exactly 0:
   // only the recurrence advances (one rescaling check per slab; power-of-two
   // rescaling is exact, so the state matches the per-group checks bit for
@@ -159,8 +162,8 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     double mx = 0.0;
 #pragma unroll 8
     for (int j = 0; j < kASlab; ++j) {
-      const double2 c = cbuf[cur][j];
-      const double pn = fma(c.x * x, p1, -(c.y * p0));
+      const double c = cbuf[cur][j];
+      const double pn = fma(c * x, p1, -p0);
       p0 = p1;
       p1 = pn;
       mx = fmax(mx, fabs(pn));
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
     const int l0 = m + kASlab * s;
     const int cur = s & 1;
     cpre = co[min(kASlab * (s + 1) + lane, lastc)];
-    const double2 *__restrict__ cb = cbuf[cur];
+    const double *__restrict__ cb = cbuf[cur];
     const bool slow = (l0 + kASlab - 1 > lmax) || wave_any(k < 0);
     // 16 groups of 4 steps: step i -> parity i & 1, row l' = i >> 1
 #pragma unroll 2
@@ -198,8 +201,8 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       if (!slow) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const double2 c = cb[4 * g + j];
-          const double pn = fma(c.x * x, p1, -(c.y * p0));
+          const double c = cb[4 * g + j];
+          const double pn = fma(c * x, p1, -p0);
           p0 = p1;
           p1 = pn;
           v[j] = float(pn);
@@ -210,8 +213,8 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
         if (lg <= lmax) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const double2 c = cb[4 * g + j];
-            const double pn = fma(c.x * x, p1, -(c.y * p0));
+            const double c = cb[4 * g + j];
+            const double pn = fma(c * x, p1, -p0);
             p0 = p1;
             p1 = pn;
             d[j] = pn;
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(64 * kAWaves, 1) void legendre_anal_f16_kernel(
       const int col = (cg * NT + ct) * 32 + c32;
       if (l <= lmax && col < ncols)
         partial[(int64_t(rg) * nalm + base + (l - m)) * ncols + col] =
-            (m == 0 && (col & 1)) ? 0.0f : sum;
+            (m == 0 && (col & 1)) ? 0.0f : float(double(sum) * csc[base + (l - m)]);
     }
     __syncthreads();  // the sums are read before the next slab's lambda tile
   }
@@ -316,11 +319,11 @@ int launch_legendre_anal_f16(const ga_plan *p, int S, const float *F, float *par
   if (nt == 2)
     hipLaunchKernelGGL(legendre_anal_f16_kernel<2>, dim3(unsigned(nblk)), dim3(64 * kAWaves), 0,
                        stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth,
-                       p->d_aab, p->d_cm, F, partial, p->nalm);
+                       p->d_gam, p->d_csc, p->d_cm, F, partial, p->nalm, p->lam_exp16);
   else
     hipLaunchKernelGGL(legendre_anal_f16_kernel<1>, dim3(unsigned(nblk)), dim3(64 * kAWaves), 0,
                        stream, p->lmax, p->npairs, p->nrings, ncols, nrg, ncg, p->d_z, p->d_sth,
-                       p->d_aab, p->d_cm, F, partial, p->nalm);
+                       p->d_gam, p->d_csc, p->d_cm, F, partial, p->nalm, p->lam_exp16);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
