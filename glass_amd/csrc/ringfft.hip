@@ -1529,7 +1529,7 @@ int launch_blu_class(const ga_plan *p, const int32_t *list, int nlist, int max_r
 // k2 = 0, 1; then the other way), and every m = k (mod n), m <= lmax, gets
 // weight e^{-i m phi0} (d0, d1) stored as one 16-byte pair.
 template <int NT, int C, bool BLU>
-__global__ __launch_bounds__(NT, 4) void ring_anal2_kernel(
+__global__ __launch_bounds__(NT, (NT == 256 && C >= 4) ? 3 : 4) void ring_anal2_kernel(
     int lmax, const int32_t *__restrict__ ring_list, int nlist,
     const int64_t *__restrict__ ring_start, const int32_t *__restrict__ ring_n,
     const int32_t *__restrict__ ring_pnum, const int32_t *__restrict__ ring_pden,
@@ -1577,7 +1577,8 @@ __global__ __launch_bounds__(NT, 4) void ring_anal2_kernel(
       for (int c = 0; c < C; ++c) {
         const int j1 = t + c * NT;
         if (j1 < r) {
-          cplx<float> v = x[c][q0 + a];
+          // q0 is 0 or 2 (this loop is not unrolled): static indices keep x in registers
+          cplx<float> v = q0 == 0 ? x[c][a] : x[c][2 + a];
           if (BLU) {
             const float2 cc = chirp[j1];
             v = cmul(cconj(v), cplx<float>{cc.x, cc.y});
@@ -1602,13 +1603,18 @@ __global__ __launch_bounds__(NT, 4) void ring_anal2_kernel(
       for (int c = 0; c < C; ++c) {
         const int k1 = t + c * NT;
         if (k1 < r) {
+          cplx<float> v;
           if (BLU) {
             const float2 cc = chirp[k1];
-            x[c][q0 + a] = cconj(cmul(X[a * ldx + pad(k1)], cplx<float>{cc.x, cc.y}));
+            v = cconj(cmul(X[a * ldx + pad(k1)], cplx<float>{cc.x, cc.y}));
           } else {
             const int pos = lbits ? int(__brev(unsigned(k1)) >> (32 - lbits)) : 0;
-            x[c][q0 + a] = X[a * ldx + pad(pos)];
+            v = X[a * ldx + pad(pos)];
           }
+          if (q0 == 0)
+            x[c][a] = v;
+          else
+            x[c][2 + a] = v;
         }
       }
     __syncthreads();
@@ -1640,9 +1646,9 @@ __global__ __launch_bounds__(NT, 4) void ring_anal2_kernel(
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const int k1 = t + c * NT;
-      if (k1 < r) {
-        X[pad(k1)] = x[c][2 - 2 * h];
-        X[ldx + pad(k1)] = x[c][3 - 2 * h];
+      if (k1 < r) {  // h is 0 or 1 (this loop is not unrolled): static indices
+        X[pad(k1)] = h == 0 ? x[c][2] : x[c][0];
+        X[ldx + pad(k1)] = h == 0 ? x[c][3] : x[c][1];
       }
     }
     __syncthreads();
@@ -1657,8 +1663,9 @@ __global__ __launch_bounds__(NT, 4) void ring_anal2_kernel(
         if (k > lmax) continue;
         // partner bin (p1, p2): in the other half, or the bin itself (k = 0, n/2)
         const int p1 = k1 ? r - k1 : 0, p2 = k1 ? 3 - k2 : (4 - k2) & 3;
-        const cplx<float> pv = p2 == k2 ? x[c][k2] : X[(p2 - (2 - 2 * h)) * ldx + pad(p1)];
-        const cplx<float> av = x[c][k2], bv = cconj(pv);
+        const cplx<float> xv = h == 0 ? x[c][b] : x[c][2 + b];  // x[c][k2]
+        const cplx<float> pv = p2 == k2 ? xv : X[(p2 - (2 - 2 * h)) * ldx + pad(p1)];
+        const cplx<float> av = xv, bv = cconj(pv);
         const cplx<float> d0 = {0.5f * (av.re + bv.re), 0.5f * (av.im + bv.im)};
         const cplx<float> d1 = {0.5f * (av.im - bv.im), -0.5f * (av.re - bv.re)};  // (a - b) / (2i)
         // ring phase e^{-i m phi0}, m phi0 = 2 pi (m pnum mod pden) / pden, stepped by n
