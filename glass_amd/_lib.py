@@ -58,6 +58,8 @@ _SIGNATURES = {
     "ga_plan_create": [_i32, _i32, ctypes.POINTER(_vp)],
     "ga_plan_destroy": [_vp],
     "ga_plan_table_bytes": [_vp, ctypes.POINTER(_sz)],
+    "ga_plan_set_tuning": [_vp, _i32, _i32],
+    "ga_plan_get_tuning": [_vp, _i32, ctypes.POINTER(_i32)],
     "ga_plan_prepare_spin": [_vp, _i32],
     "ga_workspace_size": [_vp, _i32, _i32, ctypes.POINTER(_sz)],
     "ga_map2alm_workspace_size": [_vp, _i32, _i32, ctypes.POINTER(_sz)],
@@ -175,6 +177,15 @@ class Plan:
             with torch().cuda.device(self.device):
                 call("ga_plan_prepare_spin", self.h, int(spin))
             self._spin.add(spin)
+
+    def set_tuning(self, key: int, value: int) -> None:
+        """This plan's knob ``key`` (``TUNE_*``); launches on the plan read it."""
+        call("ga_plan_set_tuning", self.h, int(key), int(value))
+
+    def get_tuning(self, key: int) -> int:
+        v = ctypes.c_int32()
+        call("ga_plan_get_tuning", self.h, int(key), ctypes.byref(v))
+        return v.value
 
     def table_bytes(self) -> int:
         n = ctypes.c_size_t()

@@ -5,9 +5,10 @@ configurations -- the split-FP16 path with one or two 32-column tiles per
 wave, and the FP32 MFMA path -- on the engine's own alm (HIP events on the
 launch stream, median of ``reps``), keeps the fastest, and caches the choice
 per (nside, lmax, shells) in a JSON file (``GLASS_AMD_TUNE_CACHE``, default
-``~/.cache/glass_amd/legendre_tune.json``).  The choice is applied through the
-library's process-wide tuning knobs, which launches read when they are issued
-(also during graph capture).
+``~/.cache/glass_amd/legendre_tune.json``).  The choice is stored in the
+engine's plan (``ga_plan_set_tuning``): every launch on that plan reads it,
+also during graph capture, and plans of other geometries and the process
+default are left as they were.
 """
 
 from __future__ import annotations
@@ -49,37 +50,36 @@ def _save(d: dict) -> None:
         pass  # an unwritable cache only costs a re-tune
 
 
-def apply(cfg: dict) -> None:
-    _lib.call("ga_set_tuning", _lib.TUNE_MFMA_PATH, int(cfg["path"]))
-    _lib.call("ga_set_tuning", _lib.TUNE_MFMA_TILES, int(cfg["nt"]))
-    _lib.call("ga_set_tuning", _lib.TUNE_MFMA_CHUNK, int(cfg.get("chunk", 0)))
+_KEYS = (("path", _lib.TUNE_MFMA_PATH), ("nt", _lib.TUNE_MFMA_TILES),
+         ("chunk", _lib.TUNE_MFMA_CHUNK))
 
 
-def current() -> dict:
-    out = {}
-    for name, key in (("path", _lib.TUNE_MFMA_PATH), ("nt", _lib.TUNE_MFMA_TILES),
-                      ("chunk", _lib.TUNE_MFMA_CHUNK)):
-        v = _lib.ctypes.c_int32()
-        _lib.call("ga_get_tuning", key, _lib.ctypes.byref(v))
-        out[name] = v.value
-    return out
+def apply(plan, cfg: dict) -> None:
+    """Store a Legendre configuration in ``plan`` (a ``_lib.Plan``)."""
+    plan.set_tuning(_lib.TUNE_MFMA_PATH, int(cfg["path"]))
+    plan.set_tuning(_lib.TUNE_MFMA_TILES, int(cfg["nt"]))
+    plan.set_tuning(_lib.TUNE_MFMA_CHUNK, int(cfg.get("chunk", 0)))
+
+
+def current(plan) -> dict:
+    return {name: plan.get_tuning(key) for name, key in _KEYS}
 
 
 def tune_legendre(eng, candidates=CANDIDATES, reps: int = 3, use_cache: bool = True) -> dict:
     """Pick the fastest Legendre configuration for a ``LognormalShells``-like
-    engine (attributes nside, lmax, S and methods stage_fields /
-    stage_legendre) and apply it."""
+    engine (attributes nside, lmax, S, plan and methods stage_fields /
+    stage_legendre) and store it in the engine's plan."""
     T = _lib.torch()
     key = f"{eng.nside}:{eng.lmax}:{eng.S}"
     cache = _load() if use_cache else {}
     if key in cache:
-        apply(cache[key])
+        apply(eng.plan, cache[key])
         return cache[key]
     eng.stage_fields()
     times = {}
     for path, nt in candidates:
         cfg = {"path": path, "nt": nt, "chunk": 0}
-        apply(cfg)
+        apply(eng.plan, cfg)
         eng.stage_legendre()  # warm-up (code load, caches)
         ts = []
         for _ in range(reps):
@@ -94,7 +94,7 @@ def tune_legendre(eng, candidates=CANDIDATES, reps: int = 3, use_cache: bool = T
     best = min(times, key=times.get)
     path, nt = (int(x) for x in best.split("/"))
     cfg = {"path": path, "nt": nt, "chunk": 0, "ms": times[best], "all_ms": times}
-    apply(cfg)
+    apply(eng.plan, cfg)
     if use_cache:
         cache = _load()
         cache[key] = cfg

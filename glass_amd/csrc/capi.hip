@@ -90,7 +90,7 @@ extern "C" int ga_legendre_prepare_batch_f32(const ga_plan *p, int32_t S, const 
   GA_REQUIRE(S >= 1 && alm && ws, GA_ERR_INVALID, "invalid argument");
   GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
              ws_bytes, ring_bytes(p, S, 4));
-  return launch_legendre_prepare_batch(p, S, alm, ws, (hipStream_t)stream);
+  return launch_legendre_prepare_batch(p, p->tune, S, alm, ws, (hipStream_t)stream);
 }
 
 extern "C" int ga_legendre_prepare_colmax_batch_f32(const ga_plan *p, int32_t S,
@@ -101,7 +101,7 @@ extern "C" int ga_legendre_prepare_colmax_batch_f32(const ga_plan *p, int32_t S,
   GA_REQUIRE(S >= 1 && alm && colpart && nparts >= 1 && ws, GA_ERR_INVALID, "invalid argument");
   GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
              ws_bytes, ring_bytes(p, S, 4));
-  return launch_legendre_prepare_batch(p, S, alm, ws, (hipStream_t)stream, colpart, nparts);
+  return launch_legendre_prepare_batch(p, p->tune, S, alm, ws, (hipStream_t)stream, colpart, nparts);
 }
 
 extern "C" int ga_legendre_contract_batch_f32(const ga_plan *p, int32_t S, const float *alm,
@@ -110,7 +110,7 @@ extern "C" int ga_legendre_contract_batch_f32(const ga_plan *p, int32_t S, const
   GA_REQUIRE(S >= 1 && alm && ws, GA_ERR_INVALID, "invalid argument");
   GA_REQUIRE(ws_bytes >= ring_bytes(p, S, 4), GA_ERR_WORKSPACE, "workspace %zu < %zu bytes",
              ws_bytes, ring_bytes(p, S, 4));
-  return launch_legendre_contract_batch(p, S, alm, ws, (hipStream_t)stream);
+  return launch_legendre_contract_batch(p, p->tune, S, alm, ws, (hipStream_t)stream);
 }
 
 extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float *F, float *maps,
@@ -214,7 +214,7 @@ extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t
              ws_bytes, ring_bytes(p, 2 * S, 4));
   GA_REQUIRE(spin == 1 || spin == 2, GA_ERR_UNSUPPORTED, "spin %d not supported (1 or 2)", spin);
   hipStream_t s = (hipStream_t)stream;
-  if (tuning().spin_path == 1) {
+  if (p->tune.spin_path == 1) {
     GA_REQUIRE(B, GA_ERR_INVALID, "the VALU spin path needs almB");
     int rc = launch_spin_synth_f32(p, spin, S, E, B, (float *)ws, s);
     if (rc) return rc;

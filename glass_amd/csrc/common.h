@@ -43,7 +43,9 @@ void clear_error();
     }                                                                          \
   } while (0)
 
-// ---- tuning knobs (set through ga_set_tuning; read by the launchers) ----
+// ---- tuning knobs: a process default (ga_set_tuning) copied into each plan
+// at creation, then per plan (ga_plan_set_tuning); launchers read the plan's
+// set, once per entry-point call ----
 struct ga_tuning {
   int mfma_nt = 0;      // 32-column tiles per wave in the MFMA Legendre stage (0 = auto; 4 = wave pairs)
   int mfma_chunk = 16;  // quads per MFMA accumulation chunk (0 = single accumulator)
@@ -52,7 +54,7 @@ struct ga_tuning {
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
 };
-ga_tuning &tuning();
+ga_tuning &tuning();  // the process default
 
 // l rows per slab of the FP16-split MFMA Legendre stage (K = 16 per parity)
 constexpr int kSlab = 32;
@@ -84,6 +86,7 @@ struct ga_plan {
   int32_t nside, lmax, nrings, npairs;
   int64_t npix, nalm;
   int device;
+  ga::ga_tuning tune;  // this plan's kernel choices
   // per ring pair (device): cos theta and sin theta of the north ring
   double *d_z = nullptr, *d_sth = nullptr;
   // per ring (device): first pixel, pixel count, phi0 = 2 pi pnum / pden

@@ -9,7 +9,8 @@ namespace ga {
 int launch_legendre_synth_f64(const ga_plan *p, const double *alm, double *F, hipStream_t s);
 
 // Legendre stage, FP32 MFMA batched: alm [idx][S] complex64 -> F (grouped ring buffer)
-int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float *F, hipStream_t s);
+int launch_legendre_synth_mfma(const ga_plan *p, const ga_tuning &tu, int S, const float *alm,
+                               float *F, hipStream_t s);
 // Legendre stage, split-FP16 MFMA batched: same in/out, in two launches:
 // operand preparation (alm -> split, swizzled B operands and column
 // exponents: the bws / colexp parts of the FP32 workspace, f32_workspace) and
@@ -23,14 +24,16 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
                                 const float *colpart = nullptr, int64_t nparts = 0,
                                 const double *csc = nullptr);
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
-                                 const int32_t *colexp, hipStream_t s);
-// the batched Legendre stage on the path chosen by the tuning knob; ws is the
-// FP32 workspace with F at its start (prepare + contract = synth)
-int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                  hipStream_t s, const float *colpart = nullptr,
+                                 const int32_t *colexp, hipStream_t s, int nt);
+// the batched Legendre stage on the path chosen by the tuning set tu (a
+// snapshot of the plan's, the same for the preparation and the contraction
+// of one synthesis); ws is the FP32 workspace with F at its start
+// (prepare + contract = synth, which snapshots p->tune once)
+int launch_legendre_prepare_batch(const ga_plan *p, const ga_tuning &tu, int S, const float *alm,
+                                  void *ws, hipStream_t s, const float *colpart = nullptr,
                                   int64_t nparts = 0);
-int launch_legendre_contract_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                   hipStream_t s);
+int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
+                                   const float *alm, void *ws, hipStream_t s);
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
                                 hipStream_t s);
 

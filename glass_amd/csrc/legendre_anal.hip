@@ -237,7 +237,7 @@ template <typename T>
 static int launch_anal(const ga_plan *p, int S, const T *F, T *alm, T *partial, bool accumulate,
                        hipStream_t stream) {
   const int nw = anal_partial_slots(p);
-  if (sizeof(T) == 4 && tuning().anal_path != 1) {
+  if (sizeof(T) == 4 && p->tune.anal_path != 1) {
     int rc = launch_legendre_anal_f16(p, S, (const float *)F, (float *)partial, stream);
     if (rc) return rc;
   } else {

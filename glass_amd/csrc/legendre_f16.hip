@@ -640,12 +640,10 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 }
 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
-                                 const int32_t *colexp, hipStream_t stream) {
+                                 const int32_t *colexp, hipStream_t stream, int nt) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
-  const ga_tuning &tu = tuning();
-  int nt = tu.mfma_nt;
   if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : 2;
   if (nt == 4) {
     // producer / consumer wave pairs over 64-column groups
@@ -678,9 +676,10 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
   return GA_OK;
 }
 
-int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                  hipStream_t stream, const float *colpart, int64_t nparts) {
-  if (tuning().mfma_path == 1) return GA_OK;  // the FP32 path reads the alm directly
+int launch_legendre_prepare_batch(const ga_plan *p, const ga_tuning &tu, int S, const float *alm,
+                                  void *ws, hipStream_t stream, const float *colpart,
+                                  int64_t nparts) {
+  if (tu.mfma_path == 1) return GA_OK;  // the FP32 path reads the alm directly
   const f32_ws_layout w = f32_workspace(p, S);
   char *base = static_cast<char *>(ws);
   return launch_legendre_prepare_f16(p, S, alm, base + w.b_off,
@@ -688,21 +687,23 @@ int launch_legendre_prepare_batch(const ga_plan *p, int S, const float *alm, voi
                                      colpart, nparts, p->d_csc);
 }
 
-int launch_legendre_contract_batch(const ga_plan *p, int S, const float *alm, void *ws,
-                                   hipStream_t stream) {
+int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
+                                   const float *alm, void *ws, hipStream_t stream) {
   float *F = static_cast<float *>(ws);
-  if (tuning().mfma_path == 1) return launch_legendre_synth_mfma(p, S, alm, F, stream);
+  if (tu.mfma_path == 1) return launch_legendre_synth_mfma(p, tu, S, alm, F, stream);
   const f32_ws_layout w = f32_workspace(p, S);
   const char *base = static_cast<const char *>(ws);
   return launch_legendre_contract_f16(p, S, F, base + w.b_off,
-                                      reinterpret_cast<const int32_t *>(base + w.e_off), stream);
+                                      reinterpret_cast<const int32_t *>(base + w.e_off), stream,
+                                      tu.mfma_nt);
 }
 
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
                                 hipStream_t stream) {
-  int rc = launch_legendre_prepare_batch(p, S, alm, ws, stream);
+  const ga_tuning tu = p->tune;  // one snapshot for both launches
+  int rc = launch_legendre_prepare_batch(p, tu, S, alm, ws, stream);
   if (rc) return rc;
-  return launch_legendre_contract_batch(p, S, alm, ws, stream);
+  return launch_legendre_contract_batch(p, tu, S, alm, ws, stream);
 }
 
 }  // namespace ga

@@ -247,10 +247,9 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_mfma_kernel(
 
 }  // namespace
 
-int launch_legendre_synth_mfma(const ga_plan *p, int S, const float *alm, float *F,
-                               hipStream_t stream) {
+int launch_legendre_synth_mfma(const ga_plan *p, const ga_tuning &tu, int S, const float *alm,
+                               float *F, hipStream_t stream) {
   const int ncols = 2 * S;
-  const ga_tuning &tu = tuning();
   int nt = tu.mfma_nt;
   if (nt != 1 && nt != 2) nt = ncols <= 32 ? 1 : 2;
   const int chunk = tu.mfma_chunk;

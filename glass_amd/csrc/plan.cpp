@@ -182,51 +182,81 @@ using namespace ga;
 extern "C" const char *ga_last_error(void) { return g_err.c_str(); }
 extern "C" const char *ga_version(void) { return "glass_amd 0.1 (gfx950)"; }
 
-extern "C" int ga_set_tuning(int32_t key, int32_t value) {
-  clear_error();
+namespace ga {
+namespace {
+// one knob of a tuning set, by key; value checks shared by the process
+// default and the per-plan setters
+int tuning_set(ga_tuning &t, int32_t key, int32_t value) {
   switch (key) {
     case GA_TUNE_MFMA_TILES:
       GA_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4, GA_ERR_INVALID,
                  "mfma tiles must be 0, 1, 2 or 4");
-      tuning().mfma_nt = value;
+      t.mfma_nt = value;
       return GA_OK;
     case GA_TUNE_MFMA_CHUNK:
       GA_REQUIRE(value >= 0 && value <= 4096, GA_ERR_INVALID, "mfma chunk out of range");
-      tuning().mfma_chunk = value;
+      t.mfma_chunk = value;
       return GA_OK;
     case GA_TUNE_MFMA_PATH:
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "mfma path must be 0, 1 or 2");
-      tuning().mfma_path = value;
+      t.mfma_path = value;
       return GA_OK;
     case GA_TUNE_RING_PATH:
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "ring path must be 0, 1 or 2");
-      tuning().ring_path = value;
+      t.ring_path = value;
       return GA_OK;
     case GA_TUNE_SPIN_PATH:
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "spin path must be 0, 1 or 2");
-      tuning().spin_path = value;
+      t.spin_path = value;
       return GA_OK;
     case GA_TUNE_ANAL_PATH:
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "analysis path must be 0, 1 or 2");
-      tuning().anal_path = value;
+      t.anal_path = value;
       return GA_OK;
     default:
       GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }
 
-extern "C" int ga_get_tuning(int32_t key, int32_t *value) {
-  clear_error();
+int tuning_get(const ga_tuning &t, int32_t key, int32_t *value) {
   GA_REQUIRE(value, GA_ERR_INVALID, "NULL value");
   switch (key) {
-    case GA_TUNE_MFMA_TILES: *value = tuning().mfma_nt; return GA_OK;
-    case GA_TUNE_MFMA_CHUNK: *value = tuning().mfma_chunk; return GA_OK;
-    case GA_TUNE_MFMA_PATH: *value = tuning().mfma_path; return GA_OK;
-    case GA_TUNE_RING_PATH: *value = tuning().ring_path; return GA_OK;
-    case GA_TUNE_SPIN_PATH: *value = tuning().spin_path; return GA_OK;
-    case GA_TUNE_ANAL_PATH: *value = tuning().anal_path; return GA_OK;
+    case GA_TUNE_MFMA_TILES: *value = t.mfma_nt; return GA_OK;
+    case GA_TUNE_MFMA_CHUNK: *value = t.mfma_chunk; return GA_OK;
+    case GA_TUNE_MFMA_PATH: *value = t.mfma_path; return GA_OK;
+    case GA_TUNE_RING_PATH: *value = t.ring_path; return GA_OK;
+    case GA_TUNE_SPIN_PATH: *value = t.spin_path; return GA_OK;
+    case GA_TUNE_ANAL_PATH: *value = t.anal_path; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
+}
+
+std::mutex g_default_mu;
+}  // namespace
+}  // namespace ga
+
+extern "C" int ga_set_tuning(int32_t key, int32_t value) {
+  clear_error();
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  return tuning_set(tuning(), key, value);
+}
+
+extern "C" int ga_get_tuning(int32_t key, int32_t *value) {
+  clear_error();
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  return tuning_get(tuning(), key, value);
+}
+
+extern "C" int ga_plan_set_tuning(ga_plan *p, int32_t key, int32_t value) {
+  clear_error();
+  GA_REQUIRE(p != nullptr, GA_ERR_INVALID, "plan is NULL");
+  return tuning_set(p->tune, key, value);
+}
+
+extern "C" int ga_plan_get_tuning(const ga_plan *p, int32_t key, int32_t *value) {
+  clear_error();
+  GA_REQUIRE(p != nullptr, GA_ERR_INVALID, "plan is NULL");
+  return tuning_get(p->tune, key, value);
 }
 
 extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
@@ -242,6 +272,10 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
   p->npairs = 2 * nside;
   p->npix = npix_of(nside);
   p->nalm = nalm_of(lmax);
+  {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    p->tune = tuning();  // the process default at creation; per plan from here on
+  }
   (void)hipGetDevice(&p->device);
   int rc = GA_OK;
   auto fail = [&](int code) { ga_plan_destroy(p); return code; };

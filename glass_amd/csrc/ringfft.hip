@@ -1902,9 +1902,9 @@ int launch_ring_synth_f64(const ga_plan *p, int S, int G, const double *F, doubl
 }
 int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
                           const ga_epilogue *epi, hipStream_t s) {
-  if (tuning().ring_path != 1) {
+  if (p->tune.ring_path != 1) {
     // the second form needs whole units of 2P shells in one group
-    const int rc = launch_ring_synth2_f32(p, S, G, F, maps, epi, tuning().ring_path != 2, s);
+    const int rc = launch_ring_synth2_f32(p, S, G, F, maps, epi, p->tune.ring_path != 2, s);
     if (rc != GA_ERR_UNSUPPORTED) return rc;
     clear_error();
   }
@@ -1916,8 +1916,8 @@ int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float 
 // and subtracts separately
 int launch_ring_resid_f32(const ga_plan *p, int S, int G, const float *F, float *out,
                           const float *ref, hipStream_t s) {
-  if (tuning().ring_path == 1) return GA_ERR_UNSUPPORTED;
-  const int rc = launch_ring_synth2_f32(p, S, G, F, out, nullptr, tuning().ring_path != 2, s, ref);
+  if (p->tune.ring_path == 1) return GA_ERR_UNSUPPORTED;
+  const int rc = launch_ring_synth2_f32(p, S, G, F, out, nullptr, p->tune.ring_path != 2, s, ref);
   if (rc == GA_ERR_UNSUPPORTED) clear_error();
   return rc;
 }
@@ -1925,7 +1925,7 @@ int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F,
   return launch_ring<double>(p, S, S, maps, F, nullptr, true, s);
 }
 int launch_ring_anal_f32(const ga_plan *p, int S, const float *maps, float *F, hipStream_t s) {
-  if (tuning().ring_path != 1) {
+  if (p->tune.ring_path != 1) {
     const int rc = launch_ring_anal2_f32(p, S, maps, F, s);
     if (rc != GA_ERR_UNSUPPORTED) return rc;
     clear_error();

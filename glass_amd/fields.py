@@ -263,13 +263,23 @@ def generate(fields, gls, nside: int, *, ncorr: int | None = None, rng=None,
     Extra keywords: ``precision`` ("fp64" plain path, default; "fp32" batched
     FP32-MFMA path with the transformation fused) and ``batch`` (shells per
     device batch in the fp32 path).
+
+    With ``precision="fp32"`` and host normals (``rng`` None or a numpy
+    Generator), the normals of a whole batch of shells are drawn from ``rng``
+    when the batch's first map is requested, i.e. ahead of the maps that are
+    yielded later.  A caller that draws from the same ``rng`` between maps
+    (as the reference's examples do for galaxy positions) therefore gets a
+    different interleaving of the stream than the reference; pass ``batch=1``
+    to keep the reference's draw order exactly, or use ``DeviceRNG``.
     """
+    # a generator function, like the reference: the checks below run at the
+    # first next(), not at the call (glass/fields.py:884-887)
     n = len(fields)
     if len(gls) != n * (n + 1) // 2:
         raise ValueError("mismatch between number of fields and gls")
     if precision not in ("fp64", "fp32"):
         raise ValueError(f"unknown precision {precision!r}")
-    return _generate(fields, gls, nside, ncorr, rng, precision, batch, fuse=True)
+    yield from _generate(fields, gls, nside, ncorr, rng, precision, batch, fuse=True)
 
 
 def _generate_grf(gls, nside: int, *, ncorr: int | None = None, rng=None,

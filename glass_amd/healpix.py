@@ -97,6 +97,18 @@ def synth_batch_f32(alm_batch, nside: int, lmax: int, epilogues=None, out=None):
     return out
 
 
+def _check_alm_sizes(lst, lmax: int | None) -> int:
+    """lmax of a list of 1-D alm (from the first if not given); ValueError
+    unless every alm has exactly nalm(lmax) entries."""
+    shapes = [tuple(getattr(a, "shape", np.shape(a))) for a in lst]
+    if not shapes or any(len(sh) != 1 for sh in shapes):
+        raise ValueError("alm must be one-dimensional")
+    lm = lmax if lmax is not None else lmax_from_nalm(shapes[0][0])
+    if any(sh[0] != nalm(lm) for sh in shapes):
+        raise ValueError("alm size does not match lmax (mmax = lmax)")
+    return int(lm)
+
+
 def alm2map(alms, nside: int, *, inplace: bool = False, lmax: int | None = None,
             pixwin: bool = False, pol: bool = True, precision: str = "fp64"):
     """glass/healpix.py:38-78 (healpy.alm2map, RING output, mmax = lmax).
@@ -117,6 +129,11 @@ def alm2map(alms, nside: int, *, inplace: bool = False, lmax: int | None = None,
         q, u = alm2map_spin(lst[1:], nside, 2, lmax if lmax is not None else
                             lmax_from_nalm(lst[1].shape[-1]), precision=precision)
         return [t, q, u]
+    if precision not in ("fp32", "fp64"):
+        raise ValueError(f"unknown precision {precision!r}")
+    # every alm must hold exactly nalm(lmax) coefficients (mmax = lmax): the
+    # kernels read that many; checked on the host, before anything is launched
+    lm = _check_alm_sizes(lst, lmax)
     T = _lib.torch()
     outs = []
     if precision == "fp32":
@@ -125,19 +142,13 @@ def alm2map(alms, nside: int, *, inplace: bool = False, lmax: int | None = None,
         for a in lst:
             d, kind = _lib.to_device(a, T.complex128)
             devs.append(d)
-        lm = lmax if lmax is not None else lmax_from_nalm(devs[0].shape[-1])
         batch = T.stack(devs, dim=1).to(T.complex64).contiguous()
         maps = synth_batch_f32(batch, nside, lm)
         outs = [_lib.from_device(maps[i], kind) for i in range(len(lst))]
-    elif precision == "fp64":
+    else:
         for a in lst:
             d, kind = _lib.to_device(a, T.complex128)
-            lm = lmax if lmax is not None else lmax_from_nalm(d.shape[-1])
-            if lm != lmax_from_nalm(d.shape[-1]):
-                raise ValueError("lmax smaller than the alm size is not supported (mmax = lmax)")
             outs.append(_lib.from_device(_synth_f64(d, nside, lm), kind))
-    else:
-        raise ValueError(f"unknown precision {precision!r}")
     return outs[0] if single else outs
 
 
@@ -210,10 +221,11 @@ def alm2map_spin(alms, nside: int, spin: int, lmax: int, precision: str = "fp64"
     T = _lib.torch()
     if int(spin) not in (1, 2):
         raise NotImplementedError(f"spin {spin} is not supported (1 or 2)")
+    if len(alms) != 2:
+        raise ValueError("alm2map_spin takes two alm, [E, B]")
+    _check_alm_sizes(list(alms), int(lmax))  # E and B are both read as nalm(lmax) values
     e, kind = _lib.to_device(alms[0], T.complex128)
     b, _ = _lib.to_device(alms[1], T.complex128)
-    if lmax_from_nalm(e.shape[-1]) != lmax:
-        raise ValueError("alm size does not match lmax (mmax = lmax)")
     plan = _lib.get_plan(nside, lmax)
     plan.prepare_spin(int(spin))
     npix = nside2npix(nside)

@@ -85,16 +85,8 @@ class UnevenBatch:
         self.graph = None
         self.nmaps = len(self.specs)
 
-    def _apply_tuning(self, key):
-        cfg = self.tuning.get(key)
-        if cfg is not None:
-            from . import autotune
-
-            autotune.apply(cfg)
-
     def run_eager(self):
         for key in self.keys:
-            self._apply_tuning(key)
             self.engines[key].run()
 
     def _branches(self):
@@ -104,7 +96,6 @@ class UnevenBatch:
             s = self.streams[key]
             s.wait_stream(main)
             with T.cuda.stream(s):
-                self._apply_tuning(key)
                 self.engines[key].run()
         for key in self.keys:
             main.wait_stream(self.streams[key])

@@ -50,8 +50,13 @@ typedef struct ga_epilogue {
 
 typedef struct ga_plan ga_plan;
 
-/* Process-wide tuning knobs of the batched MFMA Legendre stage (set by the
- * autotuner; no counterpart in the reference, which has no device code). */
+/* Tuning knobs: which kernel form each batched FP32 stage runs (chosen by
+ * the autotuner; no counterpart in the reference, which has no device code).
+ * Each plan carries its own set, copied at ga_plan_create from the process
+ * default; ga_plan_set_tuning changes one plan's set (a set-up step: not
+ * concurrently with launches on that plan), and every entry point reads its
+ * plan's set once per call.  ga_set_tuning / ga_get_tuning read and write the
+ * process default, which only affects plans created afterwards. */
 enum ga_tune_key {
   GA_TUNE_MFMA_TILES = 1, /* 32-column tiles per wave: 1, 2, 0 = automatic, or 4 =
                              producer / consumer wave pairs (split FP16 only) */
@@ -81,6 +86,9 @@ const char *ga_version(void);
  * healpy.map2alm (glass/healpix.py:71, :270). */
 int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out);
 int ga_plan_destroy(ga_plan *plan);
+/* One plan's tuning knobs (see enum ga_tune_key). */
+int ga_plan_set_tuning(ga_plan *plan, int32_t key, int32_t value);
+int ga_plan_get_tuning(const ga_plan *plan, int32_t key, int32_t *value);
 /* Bytes of device memory held by the plan's tables. */
 int ga_plan_table_bytes(const ga_plan *plan, size_t *bytes);
 

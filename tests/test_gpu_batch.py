@@ -5,6 +5,8 @@ Tolerances (max |error| / RMS of the reference output, per map):
   FP32 batched maps / shear vs the FP64 GPU path <= 1e-5
 """
 
+import contextlib
+
 import numpy as np
 import pytest
 import torch
@@ -16,6 +18,20 @@ pytestmark = pytest.mark.gpu
 from glass_amd import healpix as hp  # noqa: E402
 from glass_amd import lensing  # noqa: E402
 from glass_amd.batch import LognormalShells, ShearShells, shear_factor, shell_spectra  # noqa: E402
+
+
+@contextlib.contextmanager
+def plan_knob(nside, lmax, key, value):
+    """Set one tuning knob of the (nside, lmax) plan for the block, then restore it."""
+    from glass_amd import _lib
+
+    plan = _lib.get_plan(nside, lmax)
+    old = plan.get_tuning(key)
+    plan.set_tuning(key, value)
+    try:
+        yield
+    finally:
+        plan.set_tuning(key, old)
 
 
 def relerr(got, ref):
@@ -143,16 +159,11 @@ def test_ring_paths_agree(nside, lmax):
     gls = shell_spectra(S, lmax, rho=(0.6, 0.3, 0.1))
     eng = LognormalShells(gls, nside, ncorr=3, seed=5, lmax=lmax)
     out = {}
-    v = _lib.ctypes.c_int32()
-    _lib.call("ga_get_tuning", _lib.TUNE_RING_PATH, _lib.ctypes.byref(v))
-    try:
-        for path in (0, 1, 2):
-            _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, path)
+    for path in (0, 1, 2):
+        with plan_knob(nside, lmax, _lib.TUNE_RING_PATH, path):
             eng.run()
             torch.cuda.synchronize()
             out[path] = eng.maps.clone()
-    finally:
-        _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, v.value)
     for s in range(S):
         for path in (0, 2):
             assert relerr(out[path][s].cpu().numpy(), out[1][s].cpu().numpy()) < 2e-6, (path, s)
@@ -176,16 +187,11 @@ def test_wide_ring_kernel_matches_fused_passes(nside, lmax, S):
     gls = shell_spectra(S, lmax, rho=(0.6, 0.3, 0.1))
     eng = LognormalShells(gls, nside, ncorr=3, seed=11, lmax=lmax)
     out = {}
-    v = _lib.ctypes.c_int32()
-    _lib.call("ga_get_tuning", _lib.TUNE_RING_PATH, _lib.ctypes.byref(v))
-    try:
-        for path in (0, 2):
-            _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, path)
+    for path in (0, 2):
+        with plan_knob(nside, lmax, _lib.TUNE_RING_PATH, path):
             eng.run()
             torch.cuda.synchronize()
             out[path] = eng.maps.clone()
-    finally:
-        _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, v.value)
     for s in range(S):
         assert relerr(out[0][s].cpu().numpy(), out[2][s].cpu().numpy()) < 2e-6, s
     ref = eng.fp64_reference_map(S - 1).cpu().numpy()
@@ -210,17 +216,13 @@ def test_spin_mfma_matches_valu_and_fp64(spin, nside, lmax, S):
     B = (rng.standard_normal((n, S)) + 1j * rng.standard_normal((n, S))) * amp[:, None]
     Ed = torch.as_tensor(E).to(torch.complex64).cuda().contiguous()
     Bd = torch.as_tensor(B).to(torch.complex64).cuda().contiguous()
-    v = _lib.ctypes.c_int32()
-    _lib.call("ga_get_tuning", _lib.TUNE_SPIN_PATH, _lib.ctypes.byref(v))
     out = {}
-    try:
-        for path in (1, 2):
-            _lib.call("ga_set_tuning", _lib.TUNE_SPIN_PATH, path)
+    for path in (1, 2):
+        with plan_knob(nside, lmax, _lib.TUNE_SPIN_PATH, path):
             out[path, "eb"] = spin_batch_f32(Ed, Bd, nside, spin, lmax).cpu().numpy()
             out[path, "e0"] = spin_batch_f32(Ed, torch.zeros_like(Ed), nside, spin, lmax).cpu().numpy()
+    with plan_knob(nside, lmax, _lib.TUNE_SPIN_PATH, 2):
         out[2, "none"] = spin_batch_f32(Ed, None, nside, spin, lmax).cpu().numpy()
-    finally:
-        _lib.call("ga_set_tuning", _lib.TUNE_SPIN_PATH, v.value)
     assert np.array_equal(out[2, "none"], out[2, "e0"]) or \
         relerr(out[2, "none"], out[2, "e0"]) < 1e-6
     for kind in ("eb", "e0"):
@@ -247,15 +249,10 @@ def test_analysis_mfma_matches_valu_and_fp64(nside, lmax, S, niter):
     rng = np.random.default_rng(nside + S)
     maps = torch.as_tensor(rng.standard_normal((S, 12 * nside * nside))).cuda()
     m32 = maps.float().contiguous()
-    v = _lib.ctypes.c_int32()
-    _lib.call("ga_get_tuning", _lib.TUNE_ANAL_PATH, _lib.ctypes.byref(v))
     out = {}
-    try:
-        for path in (1, 2):
-            _lib.call("ga_set_tuning", _lib.TUNE_ANAL_PATH, path)
+    for path in (1, 2):
+        with plan_knob(nside, lmax, _lib.TUNE_ANAL_PATH, path):
             out[path] = hp.map2alm_batch_f32(m32, lmax, niter).T.cpu().numpy()
-    finally:
-        _lib.call("ga_set_tuning", _lib.TUNE_ANAL_PATH, v.value)
     for s in range(S):
         a1, a2 = out[1][s], out[2][s]
         scale = np.sqrt(np.mean(np.abs(a1) ** 2))
@@ -279,15 +276,10 @@ def test_ring_analysis_forms_agree(nside, lmax, S, niter):
     rng = np.random.default_rng(3 * nside + S)
     maps = torch.as_tensor(rng.standard_normal((S, 12 * nside * nside))).cuda()
     m32 = maps.float().contiguous()
-    v = _lib.ctypes.c_int32()
-    _lib.call("ga_get_tuning", _lib.TUNE_RING_PATH, _lib.ctypes.byref(v))
     out = {}
-    try:
-        for path in (0, 1):
-            _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, path)
+    for path in (0, 1):
+        with plan_knob(nside, lmax, _lib.TUNE_RING_PATH, path):
             out[path] = hp.map2alm_batch_f32(m32, lmax, niter).T.cpu().numpy()
-    finally:
-        _lib.call("ga_set_tuning", _lib.TUNE_RING_PATH, v.value)
     for s in range(S):
         scale = np.sqrt(np.mean(np.abs(out[1][s]) ** 2))
         assert np.max(np.abs(out[0][s] - out[1][s])) / scale < 2e-6, s

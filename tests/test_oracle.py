@@ -407,3 +407,65 @@ def test_philox_known_answers():
     assert abs(z.real.mean()) < 0.02 and abs(z.imag.mean()) < 0.02
     assert abs(z.real.var() - 1) < 0.02 and abs(z.imag.var() - 1) < 0.02
     assert abs(np.mean(z.real * z.imag)) < 0.02
+
+
+class _UnitDrawRng:
+    """Stand-in generator for the window test: the ``hot``-th call of
+    standard_normal returns real parts 1 and imaginary parts 0, every other
+    call zeros, so each yielded alm is the coefficient of that one draw."""
+
+    def __init__(self, hot: int):
+        self.hot, self.calls = hot, 0
+
+    def standard_normal(self, shape):
+        out = np.zeros(shape)
+        if self.calls == self.hot:
+            out[:, 0] = 1.0
+        self.calls += 1
+        return out
+
+
+@pytest.mark.parametrize("nfields,ncorr", [(5, 2), (6, 3), (4, 1), (5, 0)])
+def test_generate_window_implies_the_banded_covariance(nfields, ncorr):
+    """The multi-shell window of glass/fields.py:401-425 (deque of the last
+    k + 1 normal vectors, offset mis, alm = sum_i w[:, i + mis] z_i): with the
+    draws as unit vectors, the yielded alm give the coefficient matrix L_l of
+    every shell on every draw; 2 L L^T (E|z|^2 = 2 for complex normals) must be
+    the target covariance C_l(t, u) inside the band |t - u| <= ncorr and zero
+    outside it.  A wrong window offset, deque order or trimming breaks this
+    for shells past ncorr + 1."""
+    nl = 7
+    ell = np.arange(nl, dtype=float)
+    auto = [(1.0 + 0.3 * i) * (ell + 1.0) ** -1.1 for i in range(nfields)]
+    rho = (0.55, 0.3, 0.12)
+    gls = []
+    for i in range(nfields):
+        for j in range(i, -1, -1):
+            d = i - j
+            if d == 0:
+                gls.append(auto[i])
+            elif d <= len(rho):
+                gls.append(rho[d - 1] * np.sqrt(auto[i] * auto[j]))
+            else:
+                gls.append(np.zeros(0))
+    # L[l, t, s]: coefficient of draw s in shell t, read at the (l, m = 0) slot
+    L = np.zeros((nl, nfields, nfields))
+    for s in range(nfields):
+        alms = list(O.generate_grf_alm(gls, ncorr=ncorr, rng=_UnitDrawRng(s)))
+        assert len(alms) == nfields
+        for t, a in enumerate(alms):
+            L[:, t, s] = a[:nl].real  # healpy order: (l, 0) at index l
+            # every m of a given l carries the same coefficient
+            for m in range(1, nl):
+                idx = [O.nalm(nl - 1) - O.nalm(nl - 1 - m) + (l - m) for l in range(m, nl)]
+                assert np.allclose(a[idx].real, L[m:, t, s], rtol=0, atol=1e-15)
+    K = 2.0 * np.einsum("lts,lus->ltu", L, L)
+    for t in range(nfields):
+        for u in range(nfields):
+            want = O.getcl(gls, t, u, lmax=nl - 1) if abs(t - u) <= ncorr else np.zeros(nl)
+            assert np.allclose(K[:, t, u], want, rtol=1e-12, atol=1e-15), (t, u)
+    # causal and banded: shell t uses only draws t - ncorr .. t
+    for t in range(nfields):
+        for s in range(nfields):
+            if s > t or s < t - ncorr:
+                assert np.all(L[:, t, s] == 0.0), (t, s)
