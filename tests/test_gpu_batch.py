@@ -300,16 +300,16 @@ def test_wave_pair_legendre_matches_single_wave(nside, lmax, S):
     print(f"wave pairs nside={nside} lmax={lmax} S={S}", flush=True)
     gls = shell_spectra(S, lmax, rho=(0.6, 0.3, 0.1))
     eng = LognormalShells(gls, nside, ncorr=3, seed=11, lmax=lmax)
-    saved = autotune.current()
+    saved = autotune.current(eng.plan)
     out = {}
     try:
         for nt in (2, 4):
-            autotune.apply({"path": 2, "nt": nt})
+            autotune.apply(eng.plan, {"path": 2, "nt": nt})
             eng.run()
             torch.cuda.synchronize()
             out[nt] = eng.maps.clone()
     finally:
-        autotune.apply(saved)
+        autotune.apply(eng.plan, saved)
     assert torch.equal(out[2], out[4])
     ref = eng.fp64_reference_map(S - 1).cpu().numpy()
     assert relerr(out[4][S - 1].cpu().numpy(), ref) < 1e-5
