@@ -65,6 +65,8 @@ _SIGNATURES = {
     "ga_map2alm_workspace_size": [_vp, _i32, _i32, ctypes.POINTER(_sz)],
     "ga_alm2map_f64": [_vp, _vp, _vp, _vp, _sz, _vp],
     "ga_alm2map_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp],
+    "ga_noalloc_scratch_bytes": [ctypes.POINTER(_sz)],
+    "ga_alm2map_batch_f32_noalloc": [_vp, _i32, _vp, _vp, _vp, _vp],
     "ga_legendre_synth_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_legendre_prepare_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_legendre_contract_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],

@@ -1,5 +1,7 @@
 // extern "C" entry points of the spherical-harmonic transforms: argument
 // checks, workspace carving and the kernel sequence of each transform.
+#include <mutex>
+
 #include "common.h"
 #include "device.h"
 #include "kernels.h"
@@ -71,6 +73,66 @@ extern "C" int ga_alm2map_batch_f32(const ga_plan *p, int32_t S, const float *al
              "workspace %zu < %zu bytes", ws_bytes, ring_bytes(p, S, 4));
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_legendre_synth_batch(p, S, alm, ws, s);
+  if (rc) return rc;
+  return launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)ws, maps, epi, s);
+}
+
+// ---- fixed device scratch of ga_alm2map_batch_f32_noalloc -------------------
+// One array per device (each GPU that loads the library holds its own copy
+// until the process ends).  Host code never passes the array's name to a
+// kernel: its address on the current device comes from hipGetSymbolAddress,
+// taken once per device on the entry point's first call there.
+namespace ga {
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(int dev, void **out) {
+  GA_REQUIRE(dev >= 0 && dev < kMaxDevices, GA_ERR_UNSUPPORTED, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    GA_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    GA_REQUIRE(ptr != nullptr, GA_ERR_HIP, "no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return GA_OK;
+}
+}  // namespace
+}  // namespace ga
+
+extern "C" int ga_noalloc_scratch_bytes(size_t *bytes) {
+  clear_error();
+  GA_REQUIRE(bytes, GA_ERR_INVALID, "NULL argument");
+  *bytes = kNoAllocScratchBytes;
+  return GA_OK;
+}
+
+extern "C" int ga_alm2map_batch_f32_noalloc(const ga_plan *p, int32_t S, const float *alm,
+                                            float *maps, const ga_epilogue *epi, void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && alm && maps, GA_ERR_INVALID, "invalid argument");
+  const size_t need = ring_bytes(p, S, 4);
+  GA_REQUIRE(need <= kNoAllocScratchBytes, GA_ERR_UNSUPPORTED,
+             "nside %d, lmax %d, %d maps need %zu bytes of scratch; the fixed scratch holds %zu",
+             p->nside, p->lmax, S, need, kNoAllocScratchBytes);
+  int dev = -1;
+  GA_CHECK_HIP(hipGetDevice(&dev));
+  GA_REQUIRE(dev == p->device, GA_ERR_INVALID, "plan belongs to device %d, current device is %d",
+             p->device, dev);
+  void *ws = nullptr;
+  int rc = scratch_address(dev, &ws);
+  if (rc) return rc;
+  // the kernels of ga_alm2map_batch_f32 on the fixed scratch: everything they
+  // need zeroed (the column maxima) is cleared by a fill kernel in the
+  // preparation launch, every other byte they read was written earlier in
+  // the same call
+  hipStream_t s = (hipStream_t)stream;
+  rc = launch_legendre_synth_batch(p, S, alm, ws, s);
   if (rc) return rc;
   return launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)ws, maps, epi, s);
 }

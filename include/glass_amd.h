@@ -119,6 +119,21 @@ int ga_alm2map_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
                          float *maps, const ga_epilogue *epi, void *workspace,
                          size_t ws_bytes, void *stream);
 
+/* ga_alm2map_batch_f32 without a workspace argument and without any device
+ * allocation: the workspace is a fixed scratch array in the library's device
+ * code (ga_noalloc_scratch_bytes: 128 MiB per device, held from the library's
+ * load until the process ends), for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
+ * Sizes whose workspace (ga_workspace_size(plan, nshell, 4)) exceeds the
+ * scratch are rejected with GA_ERR_UNSUPPORTED before anything is launched;
+ * the current device must be the plan's.  Every call on one GPU shares that
+ * GPU's scratch, which keeps the last call's contents: calls that overlap on
+ * one GPU (different streams, or host threads) race and must be serialised by
+ * the caller.  No counterpart in the reference. */
+int ga_noalloc_scratch_bytes(size_t *bytes);
+int ga_alm2map_batch_f32_noalloc(const ga_plan *plan, int32_t nshell, const float *alm,
+                                 float *maps, const ga_epilogue *epi, void *stream);
+
 /* The two stages of ga_alm2map_batch_f32, callable separately (for timing
  * and for callers that keep the ring buffer): the MFMA Legendre stage
  * alm [idx][S] -> ring buffer F, which occupies the start of the workspace
