@@ -9,7 +9,8 @@ meaning and error messages):
     glass_amd.healpix.alm2map / map2alm / alm2map_spin / almxfl / ...
                                                      (glass/healpix.py)
     glass_amd.grf.Normal / Lognormal / SquaredNormal (glass/grf)
-    glass_amd.from_convergence / shear_from_convergence (glass/lensing.py)
+    glass_amd.from_convergence / shear_from_convergence / MultiPlaneConvergence /
+    multi_plane_matrix / multi_plane_weights         (glass/lensing.py)
 
 All numerics run in hand-written HIP kernels for gfx950 behind the C ABI in
 include/glass_amd.h; there is no CPU fallback.
@@ -25,11 +26,19 @@ from .fields import (
     iternorm,
     nfields_from_nspectra,
 )
-from .lensing import from_convergence, shear_from_convergence
+from .lensing import (
+    MultiPlaneConvergence,
+    RadialWindow,
+    from_convergence,
+    multi_plane_matrix,
+    multi_plane_weights,
+    shear_from_convergence,
+)
 from .rng import DeviceRNG
 
 __all__ = [
-    "DeviceRNG", "cls2cov", "from_convergence", "generate", "generate_gaussian",
+    "DeviceRNG", "MultiPlaneConvergence", "RadialWindow", "cls2cov", "from_convergence", "generate", "generate_gaussian",
     "generate_lognormal", "getcl", "grf", "harmonics", "healpix", "iternorm",
+    "multi_plane_matrix", "multi_plane_weights",
     "nfields_from_nspectra", "rng", "shear_from_convergence",
 ]

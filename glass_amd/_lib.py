@@ -34,6 +34,7 @@ TUNE_MFMA_PATH = 3
 TUNE_RING_PATH = 4
 TUNE_SPIN_PATH = 5
 TUNE_ANAL_PATH = 6
+TUNE_F16_FLUSH = 7
 
 T_NORMAL = 0
 T_LOGNORMAL = 1
@@ -85,6 +86,10 @@ _SIGNATURES = {
     "ga_fields_colmax_rows": [_i32, ctypes.POINTER(_i64)],
     "ga_fields_alm_colmax": [_i32, _i32, _i32, _i32, _vp, _u64, _vp, _i32, _vp, _vp],
     "ga_transform_f64": [_i64, _vp, ctypes.POINTER(Epilogue), _vp, _vp],
+    "ga_pixwin_workspace_size": [_i32, _i32, ctypes.POINTER(_sz)],
+    "ga_pixwin": [_i32, _i32, _i32, _vp, _vp, _vp, _sz, _vp],
+    "ga_multi_plane_update": [_i64, _i32, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
+                              _vp],
 }
 
 _lib = None

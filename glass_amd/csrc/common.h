@@ -53,7 +53,17 @@ struct ga_tuning {
   int ring_path = 0;    // FP32 ring synthesis: 0 auto (fused passes, register fold on wide rings), 1 radix-4 stages, 2 fused passes only
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
+  int f16_flush = 0;    // split-FP16 synthesis accumulation: n > 0 master sums in registers
+                        // flushed every n slabs (one column tile), -k (k >= 2) k l-chunks
+                        // summed in the ring buffer, -1 one chain, 0 auto (kAutoFlushSlabs
+                        // when lmax > kAutoFlushLmax, else kAutoChunks)
 };
+// lmax above which the split-FP16 synthesis flushes its MFMA accumulators into
+// register master sums by default, the flush period in slabs, and the default
+// accumulation below that lmax
+constexpr int kAutoFlushLmax = 4096;
+constexpr int kAutoFlushSlabs = 8;
+constexpr int kAutoChunks = -2;
 ga_tuning &tuning();  // the process default
 
 // l rows per slab of the FP16-split MFMA Legendre stage (K = 16 per parity)
@@ -62,12 +72,16 @@ __host__ __device__ inline int64_t nslab_of(int64_t lmax, int64_t m) { return (l
 
 // Workspace of the FP32 batched synthesis (sizes for nbatch maps, ncols = 2
 // nbatch real columns): the ring buffer F, then the split-FP16 B operands of
-// the Legendre stage, then one int32 exponent per column.  Each part starts
-// at a multiple of 256 bytes.
+// the Legendre stage, then one int32 exponent per column, then (when the
+// plan's synthesis accumulates in l-chunks) the partial sums of every wave.
+// Each part starts at a multiple of 256 bytes.
 struct f32_ws_layout {
-  size_t f_bytes, b_off, b_bytes, e_off, total;
+  size_t f_bytes, b_off, b_bytes, e_off, p_off, p_bytes, total;
 };
 f32_ws_layout f32_workspace(const ga_plan *p, int nbatch);
+// the accumulation form the split-FP16 synthesis of plan p runs under tuning
+// set tu: the f16_flush knob with 0 (automatic) resolved
+int f16_accumulation(const ga_plan *p, const ga_tuning &tu);
 
 // ---- sizes ----------------------------------------------------------------
 inline int64_t nalm_of(int64_t lmax) { return (lmax + 1) * (lmax + 2) / 2; }

@@ -23,8 +23,12 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
                                 int32_t *colexp, hipStream_t s, int lmin = 0,
                                 const float *colpart = nullptr, int64_t nparts = 0,
                                 const double *csc = nullptr);
+// nt: column tiles per wave (0 automatic, 1, 2, 4 = wave pairs); flush: the
+// accumulation form (f16_accumulation: -1 one chain, n > 0 register master
+// sums every n slabs, -k k l-chunks with per-wave partial sums in partials)
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
-                                 const int32_t *colexp, hipStream_t s, int nt);
+                                 const int32_t *colexp, hipStream_t s, int nt, int flush,
+                                 void *partials);
 // the batched Legendre stage on the path chosen by the tuning set tu (a
 // snapshot of the plan's, the same for the preparation and the contraction
 // of one synthesis); ws is the FP32 workspace with F at its start

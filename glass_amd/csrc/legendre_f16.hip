@@ -368,14 +368,21 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
   }
 }
 
-template <int NT, int MINW>
+// FLUSH: every `flush` slabs the accumulators are added into FP32 master sums
+// and cleared, so that no accumulator chain is longer than `flush` slabs
+// (3 flush MFMA roundings) and the master sums take one rounding per flush
+// instead of 3 flush: the rounding error of the l-sums shrinks by about
+// sqrt(1 / (3 flush) + flush / nslab), which the long sums of lmax > 4096
+// need to stay within 1e-5 of the map RMS.  It doubles the accumulator
+// registers, so it is instantiated with one column tile per wave.
+template <int NT, int MINW, bool FLUSH>
 __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
     int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
     const double *__restrict__ z, const double *__restrict__ sth, const double *__restrict__ gam,
     const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
     const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
     const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F,
-    int lam_exp) {
+    int lam_exp, int flush, int nchunk, float4 *__restrict__ part) {
   // XCD-aware decode: block b runs on XCD b % 8; XCD x takes m = x, x + 8, ...
   // (largest l-range first), all (ring group, column group) tiles of one m
   // back to back so they share that m's B slabs in the XCD's L2.
@@ -410,12 +417,25 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   for (int ct = 0; ct < NT; ++ct) ctile[ct] = min(cg * NT + ct, nct - 1);
 
   f32x16 acc[2][2][NT];  // [parity][row tile][col tile]
+  f32x16 mst[2][2][FLUSH ? NT : 1];  // master sums of the flushing form
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
+      for (int ct = 0; ct < NT; ++ct) {
+        acc[p][t][ct] = f32x16{};
+        if (FLUSH) mst[p][t][FLUSH ? ct : 0] = f32x16{};
+      }
+  int since_flush = 0;  // slabs accumulated since the last flush
+  // chunked form (nchunk >= 2): the l-range of the wave is cut into nchunk
+  // chunks of slabs; at each cut the accumulators are stored to the wave's
+  // slot of that chunk in memory (16 NT float4 per lane, coalesced) and
+  // restart from zero; the stored chunks are added back, in order, before
+  // the tile is written.  Stores only at the cuts: no loads there.
+  int chunk = 0, s_first = 0, span = 0;
+  float4 *__restrict__ mypart =
+      part + (int64_t(blockIdx.x) * 4 + wave) * int64_t(nchunk - 1) * (16 * NT * 64) + lane;
 
   // Recurrence coefficients go through LDS, one slab (32 double2) per wave
   // and buffer: the next slab's are loaded from global memory (lanes 0..31,
@@ -436,34 +456,111 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   // matches the per-group checks bit for bit).  The slab in which some lane
   // reaches FP16 range is undone and run again by the full loop.
   int s = skip_dead_slabs(cbuf[wave], co, lastc, lane, x, m, lmax, nsl, p0, p1, k, cpre);
+  int cut_end = nsl;  // end of the current chunk of slabs
+  if (!FLUSH && nchunk >= 2 && nsl - s >= 4 * nchunk) {
+    s_first = s;
+    span = nsl - s;
+    cut_end = s_first + span / nchunk;
+  }
 
-  for (; s < nsl; ++s) {
-    const int l0 = m + kSlab * s;
-    const int cur = s & 1;
-    // B operands of this slab, consumed after the recurrence below
-    uint4 bh[2][NT], bl[2][NT];
+  for (;;) {
+    for (; s < cut_end; ++s) {
+      const int l0 = m + kSlab * s;
+      const int cur = s & 1;
+      // B operands of this slab, consumed after the recurrence below
+      uint4 bh[2][NT], bl[2][NT];
+  #pragma unroll
+      for (int p = 0; p < 2; ++p)
+  #pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+          const uint4 *__restrict__ u = B + (((s0 + s) * 2 + p) * nct + ctile[ct]) * 128;
+          bh[p][ct] = u[lane];
+          bl[p][ct] = u[64 + lane];
+        }
+      // next slab's coefficients (clamped past the end; never used there)
+      cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
+      const double *__restrict__ cb = cbuf[wave][cur];
+
+      uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
+      slab_values(cb, x, l0, lmax, p0, p1, k, ph, pl);
+      if (lane < kSlab) cbuf[wave][cur ^ 1][lane] = cpre;
+      __builtin_amdgcn_wave_barrier();
+
+      // MFMAs only if some lane has a nonzero lambda in this slab (k only
+      // grows, so a lane still scaled at the end of the slab contributed nothing)
+      if (wave_any(k == 0)) {
+        f16::slab_mfma<NT>(ph, pl, bh, bl, acc);
+        if (FLUSH && ++since_flush == flush) {
+  #pragma unroll
+          for (int p = 0; p < 2; ++p)
+  #pragma unroll
+            for (int t = 0; t < 2; ++t)
+  #pragma unroll
+              for (int ct = 0; ct < NT; ++ct) {
+                mst[p][t][FLUSH ? ct : 0] += acc[p][t][ct];
+                acc[p][t][ct] = f32x16{};
+              }
+          since_flush = 0;
+        }
+      }
+    }
+    if (FLUSH || s >= nsl) break;
+    // end of a chunk: store it to its slot and restart the accumulators
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < NT; ++ct) {
-        const uint4 *__restrict__ u = B + (((s0 + s) * 2 + p) * nct + ctile[ct]) * 128;
-        bh[p][ct] = u[lane];
-        bl[p][ct] = u[64 + lane];
-      }
-    // next slab's coefficients (clamped past the end; never used there)
-    cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
-    const double *__restrict__ cb = cbuf[wave][cur];
-
-    uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
-    slab_values(cb, x, l0, lmax, p0, p1, k, ph, pl);
-    if (lane < kSlab) cbuf[wave][cur ^ 1][lane] = cpre;
-    __builtin_amdgcn_wave_barrier();
-
-    // MFMAs only if some lane has a nonzero lambda in this slab (k only
-    // grows, so a lane still scaled at the end of the slab contributed nothing)
-    if (wave_any(k == 0)) f16::slab_mfma<NT>(ph, pl, bh, bl, acc);
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            mypart[((chunk * 16 * NT) + ((p * 2 + t) * NT + ct) * 4 + q) * 64] =
+                make_float4(acc[p][t][ct][4 * q], acc[p][t][ct][4 * q + 1],
+                            acc[p][t][ct][4 * q + 2], acc[p][t][ct][4 * q + 3]);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
+    ++chunk;
+    cut_end = chunk + 1 < nchunk ? s_first + (chunk + 1) * span / nchunk : nsl;
+  }
+  if (FLUSH) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] += mst[p][t][FLUSH ? ct : 0];
   }
 
+  if (!FLUSH && chunk > 0) {
+    // the stored chunks added to the last one (in registers), first to last,
+    // 16 values at a time (the scheduling barrier keeps the compiler from
+    // issuing every load at once, which would spill the accumulators)
+    for (int c = 0; c < chunk; ++c) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int ct = 0; ct < NT; ++ct) {
+            float4 o[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              o[q] = mypart[((c * 16 * NT) + ((p * 2 + t) * NT + ct) * 4 + q) * 64];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              acc[p][t][ct][4 * q] += o[q].x;
+              acc[p][t][ct][4 * q + 1] += o[q].y;
+              acc[p][t][ct][4 * q + 2] += o[q].z;
+              acc[p][t][ct][4 * q + 3] += o[q].w;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+    }
+  }
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
                  colexp, F, lam_exp);
 }
@@ -640,10 +737,16 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 }
 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
-                                 const int32_t *colexp, hipStream_t stream, int nt) {
+                                 const int32_t *colexp, hipStream_t stream, int nt, int flush,
+                                 void *partials) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
+  // flush (resolved): n > 0 master sums in registers (they take the registers
+  // of a second tile); n <= -2: -n l-chunks with partial sums in memory
+  const int nchunk = flush <= -2 ? -flush : 1;
+  GA_REQUIRE(nchunk == 1 || partials, GA_ERR_INVALID, "chunked accumulation needs partials");
+  if (flush > 0) nt = 1;
   if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : 2;
   if (nt == 4) {
     // producer / consumer wave pairs over 64-column groups
@@ -664,13 +767,15 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
   const int nm8 = (p->lmax + 1 + 7) / 8;
   const int64_t nblk2 = int64_t(8) * nm8 * nrg * ncg;
   GA_REQUIRE(nblk2 < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
-#define GA_F16_LAUNCH(NT_, MW_)                                                                \
-  hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_>), dim3(unsigned(nblk2)), dim3(256), \
-                     0, stream, p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, p->d_z,   \
-                     p->d_sth, p->d_gam, p->d_cm, p->d_ring_pnum, p->d_ring_pden,               \
-                     p->d_slab_off, B, colexp, F, p->lam_exp16)
-  if (nt == 1) GA_F16_LAUNCH(1, 2);
-  else GA_F16_LAUNCH(2, 2);
+#define GA_F16_LAUNCH(NT_, MW_, FL_)                                                            \
+  hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_, FL_>), dim3(unsigned(nblk2)),        \
+                     dim3(256), 0, stream, p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, \
+                     p->d_z, p->d_sth, p->d_gam, p->d_cm, p->d_ring_pnum, p->d_ring_pden,       \
+                     p->d_slab_off, B, colexp, F, p->lam_exp16, flush, nchunk,               \
+                     static_cast<float4 *>(partials))
+  if (flush > 0) GA_F16_LAUNCH(1, 2, true);
+  else if (nt == 1) GA_F16_LAUNCH(1, 2, false);
+  else GA_F16_LAUNCH(2, 2, false);
 #undef GA_F16_LAUNCH
   GA_LAUNCH_CHECK();
   return GA_OK;
@@ -693,9 +798,13 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
   if (tu.mfma_path == 1) return launch_legendre_synth_mfma(p, tu, S, alm, F, stream);
   const f32_ws_layout w = f32_workspace(p, S);
   const char *base = static_cast<const char *>(ws);
+  const int acc_form = f16_accumulation(p, tu);
+  GA_REQUIRE(acc_form > -2 || w.p_bytes > 0, GA_ERR_WORKSPACE,
+             "the workspace was sized for another accumulation form; size it again");
   return launch_legendre_contract_f16(p, S, F, base + w.b_off,
                                       reinterpret_cast<const int32_t *>(base + w.e_off), stream,
-                                      tu.mfma_nt);
+                                      tu.mfma_nt, acc_form,
+                                      const_cast<char *>(base) + w.p_off);
 }
 
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,

@@ -213,6 +213,10 @@ int tuning_set(ga_tuning &t, int32_t key, int32_t value) {
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "analysis path must be 0, 1 or 2");
       t.anal_path = value;
       return GA_OK;
+    case GA_TUNE_F16_FLUSH:
+      GA_REQUIRE(value >= -8 && value <= 4096, GA_ERR_INVALID, "f16 flush out of range");
+      t.f16_flush = value;
+      return GA_OK;
     default:
       GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
@@ -227,6 +231,7 @@ int tuning_get(const ga_tuning &t, int32_t key, int32_t *value) {
     case GA_TUNE_RING_PATH: *value = t.ring_path; return GA_OK;
     case GA_TUNE_SPIN_PATH: *value = t.spin_path; return GA_OK;
     case GA_TUNE_ANAL_PATH: *value = t.anal_path; return GA_OK;
+    case GA_TUNE_F16_FLUSH: *value = t.f16_flush; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }
@@ -537,7 +542,24 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
   w.b_off = w.f_bytes;
   w.b_bytes = up(size_t(p->nslab) * 2 * nct * 2048);
   w.e_off = w.b_off + w.b_bytes;
-  w.total = w.e_off + up(nct * 32 * 4 * 2);  // exponents + maxima scratch
+  w.p_off = w.e_off + up(nct * 32 * 4 * 2);  // exponents + maxima scratch
+  w.p_bytes = 0;
+  const int acc_form = f16_accumulation(p, p->tune);
+  if (acc_form <= -2) {
+    // per wave of the two-tile launch grid (8 m-groups x ceil((lmax + 1) / 8)
+    // x ring groups x 64-column groups, 4 waves each): 32 float4 per lane for
+    // each of the -acc_form - 1 stored chunks; the one-tile grid needs at
+    // most as much
+    const size_t nm8 = size_t(p->lmax + 1 + 7) / 8, nrg = size_t(p->npairs + 255) / 256;
+    const size_t ncg = size_t(2 * nbatch + 63) / 64;
+    w.p_bytes = up(8 * nm8 * nrg * ncg * 4 * 64 * 32 * 16 * size_t(-acc_form - 1));
+  }
+  w.total = w.p_off + w.p_bytes;
   return w;
+}
+
+int f16_accumulation(const ga_plan *p, const ga_tuning &tu) {
+  if (tu.f16_flush != 0) return tu.f16_flush;
+  return p->lmax > kAutoFlushLmax ? kAutoFlushSlabs : kAutoChunks;
 }
 }  // namespace ga

@@ -118,12 +118,16 @@ def alm2map(alms, nside: int, *, inplace: bool = False, lmax: int | None = None,
     other sequence gives one map per alm.
     """
     del inplace  # outputs are always fresh device buffers; inputs are never modified
-    if pixwin:
-        raise NotImplementedError(
-            "pixwin=True needs the HEALPix pixel-window tables (healpy data files), "
-            "which this build does not ship; apply the window with almxfl instead")
     single = _is_single(alms)
     lst = [alms] if single else _as_list(alms)
+    if pixwin:
+        # healpy smooths with its default-length windows (l < 3 nside; alm
+        # beyond are zeroed by almxfl): pw0 on T, pw2 on E and B
+        pw0, pw2 = (a[: 3 * int(nside)] for a in _window_tables(int(nside)))
+        polarised = pol and len(lst) == 3 and not single
+        lst = [almxfl(a, pw2 if (polarised and i > 0) else pw0) for i, a in enumerate(lst)]
+        if single:
+            alms = lst[0]
     if pol and len(lst) == 3 and not single:
         t = alm2map(lst[0], nside, lmax=lmax, pol=False, precision=precision)
         q, u = alm2map_spin(lst[1:], nside, 2, lmax if lmax is not None else
@@ -276,13 +280,52 @@ def almxfl(alm, fl, *, inplace: bool = False):
     return res
 
 
+_pixwin_cache: dict[tuple[int, int], tuple[np.ndarray, np.ndarray]] = {}
+
+
+def _window_tables(nside: int) -> tuple[np.ndarray, np.ndarray]:
+    """(pw0, pw2) for l = 0 .. 4 nside, computed once per (nside, device)."""
+    T = _lib.torch()
+    _lib.require_gpu()
+    key = (nside, T.cuda.current_device())
+    if key not in _pixwin_cache:
+        lfull = 4 * nside
+        n = _lib.ctypes.c_size_t()
+        _lib.call("ga_pixwin_workspace_size", nside, lfull, _lib.ctypes.byref(n))
+        ws = _lib.workspace(n.value)
+        pw0 = T.empty(lfull + 1, dtype=T.float64, device="cuda")
+        pw2 = T.empty(lfull + 1, dtype=T.float64, device="cuda")
+        _lib.call("ga_pixwin", nside, lfull, 2, _lib.ptr(pw0), _lib.ptr(pw2), _lib.ptr(ws),
+                  ws.numel(), _lib.stream_ptr())
+        _pixwin_cache[key] = (pw0.cpu().numpy(), pw2.cpu().numpy())
+    return _pixwin_cache[key]
+
+
 def pixwin(nside: int, *, lmax: int | None = None, pol: bool = False, xp=None):
-    """glass/healpix.py:313-356.  The pixel-window tables are healpy package
-    data (FITS) that this build cannot load; pass windows to almxfl /
-    from_convergence(pixwin_ratio=...) explicitly instead."""
+    """glass/healpix.py:313-356: the pixel window function(s) of ``nside``.
+
+    healpy reads these from tabulated package data; here they are computed on
+    the GPU from their definition (``ga_pixwin``: the harmonic power of each
+    pixel's normalised top hat averaged over the pixels; the polarisation
+    window from the spin-2 harmonics), once per nside up to l = 4 nside, and
+    sliced.  ``lmax`` defaults to 3 nside - 1 and may not exceed 4 nside (the
+    range of the tabulated windows).  Returns float64 arrays of ``xp``
+    (numpy by default); with ``pol`` the pair (temperature, polarisation).
+    A HEALPY_DATAPATH that names no directory raises as in the reference,
+    although no data file is read."""
     _get_healpy_datapath()
-    raise NotImplementedError(
-        "HEALPix pixel-window tables are not available in this build; supply them explicitly")
+    nside = int(nside)
+    if nside < 1:
+        raise ValueError(f"invalid nside {nside}")
+    if lmax is None:
+        lmax = 3 * nside - 1
+    lmax = int(lmax)
+    if lmax < 0 or lmax > 4 * nside:
+        raise ValueError(f"lmax {lmax} outside 0 .. 4 nside = {4 * nside}")
+    pw0, pw2 = (a[: lmax + 1].copy() for a in _window_tables(nside))
+    if xp is not None:
+        pw0, pw2 = xp.asarray(pw0, dtype=xp.float64), xp.asarray(pw2, dtype=xp.float64)
+    return (pw0, pw2) if pol else pw0
 
 
 def sequence_of_arrays(x) -> bool:

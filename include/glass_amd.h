@@ -68,8 +68,17 @@ enum ga_tune_key {
                              2 = synthesis by four-array fused passes on every ring */
   GA_TUNE_SPIN_PATH = 5,  /* FP32 spin synthesis: 0 = automatic (split FP16 MFMA),
                              1 = VALU, 2 = split FP16 MFMA */
-  GA_TUNE_ANAL_PATH = 6   /* FP32 analysis Legendre stage: 0 = automatic (split FP16
+  GA_TUNE_ANAL_PATH = 6,  /* FP32 analysis Legendre stage: 0 = automatic (split FP16
                              MFMA), 1 = VALU, 2 = split FP16 MFMA */
+  GA_TUNE_F16_FLUSH = 7   /* split-FP16 synthesis accumulation (shorter FP32 rounding
+                             chains): n > 0 = every n slabs of 32 l rows the MFMA
+                             accumulators are added into FP32 master sums in
+                             registers (one column tile per wave); -k (k = 2..8) =
+                             the l-range cut into k chunks, the first k - 1 parked
+                             in the workspace and added back at the end (the
+                             workspace grows: size it under the same setting);
+                             -1 = one chain; 0 = automatic (8 when lmax > 4096,
+                             else -2) */
 };
 int ga_set_tuning(int32_t key, int32_t value);
 int ga_get_tuning(int32_t key, int32_t *value);
@@ -242,6 +251,26 @@ int ga_fields_colmax_rows(int32_t nl, int64_t *rows);
 int ga_fields_alm_colmax(int32_t nl, int32_t k, int32_t first_shell, int32_t nshell,
                          const double *w, uint64_t seed, float *out_f32, int32_t ldo,
                          float *colpart, void *stream);
+
+/* HEALPix pixel window functions for l = 0 .. lmax, computed from their
+ * definition (the harmonic power of each pixel's normalised top hat, averaged
+ * over the pixels; spin s from the spin-s harmonics): pw0 the scalar window,
+ * pws (spin 1 or 2, NULL for spin 0) the spin-s window, 0 for l < s.  float64
+ * device outputs; nside <= 8192.  Stands in for healpy.pixwin, called at
+ * glass/healpix.py:313-356 (pol = True: spin 2) and used by
+ * glass/lensing.py:353-362, :414-419 (discretized = True). */
+int ga_pixwin_workspace_size(int32_t nside, int32_t lmax, size_t *bytes);
+int ga_pixwin(int32_t nside, int32_t lmax, int32_t spin, double *pw0, double *pws,
+              void *workspace, size_t ws_bytes, void *stream);
+
+/* Multi-plane convergence step on n-pixel maps (real_bytes 4 = float32,
+ * 8 = float64): kappa3 = (1 - t) kappa1 + t kappa2 + f delta2, evaluated as
+ * the reference's three in-place float64 array operations (same roundings;
+ * float32 maps rounded back after each); delta2 NULL = 0 (before the first
+ * mass plane); kappa3 may alias kappa1.  t and f come from the cosmology on
+ * the host.  glass/lensing.py:511-586 MultiPlaneConvergence.add_plane. */
+int ga_multi_plane_update(int64_t n, int32_t real_bytes, const void *kappa1, const void *kappa2,
+                          const void *delta2, double t, double f, void *kappa3, void *stream);
 
 /* Pointwise transforms on a float64 map (n pixels), out may equal x.
  * glass/grf/_transformations.py. */

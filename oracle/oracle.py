@@ -45,8 +45,20 @@ Pinning (what checks each function, all in ``tests/test_oracle.py``):
                     generator of this build (not the reference's numpy stream)
                     is then checked against device_normals built on it.
 
-Parity unpinned: healpy's pixel-weight files (use_pixel_weights=True) and pixwin
-tables are not available, so those inputs are taken from the caller.
+* MultiPlaneConvergence -- the reference's own identities
+                    (tests/core/test_lensing.py:64-120): multi_plane_matrix is lower
+                    triangular and reproduces the convergence sequence of random
+                    mass planes; multi_plane_weights reproduces weighted sums.
+* pixwin         -- pixwin_definition evaluates the definition directly (pixel
+                    averages of the spin-weighted harmonics from Wigner's explicit
+                    d formula, every pixel, high-order quadrature); pixwin_pairs
+                    (the pair form the device computes, exact per pair, over the
+                    symmetry-reduced pixel set) is checked against it, against
+                    itself over all pixels, and against w_0 = 1.
+
+Parity unpinned: healpy's pixel-weight files (use_pixel_weights=True) and the
+tabulated pixel windows that healpy ships (pixwin here is computed from its
+definition; the tables themselves are not available to compare with).
 """
 
 from __future__ import annotations
@@ -454,3 +466,219 @@ def device_normals(seed: int, shell: int, g: np.ndarray) -> np.ndarray:
     u2 = (w[..., 1] >> 8).astype(np.float64) * 2.0**-24
     rad = np.sqrt(-2.0 * np.log(u1))
     return rad * np.cos(2 * np.pi * u2) + 1j * rad * np.sin(2 * np.pi * u2)
+
+
+# --------------------------------------------------------------------------
+# HEALPix pixel windows from their definition (restates what healpy.pixwin
+# tabulates, glass/healpix.py:313-356; the tables are not available here).
+#
+# w^s_l = sqrt(mean_p (4 pi / (2l+1)) sum_m |<sY_lm>_p|^2), <.>_p the average
+# over pixel p.  The pair form: (4 pi/(2l+1)) sum_m sY_lm(a) sY_lm(b)^* =
+# d^l_ss(beta_ab) e^{i s psi_ab}, so the power is the pair average of
+# d^l_ss(beta) cos(s psi).
+
+# base-face offsets of the HEALPix grid (Gorski et al. 2005): jr = jrll - x - y
+# in units of nside rings from the north pole, phi offset jpll in units of pi/4
+_JRLL = (2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4)
+_JPLL = (1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7)
+
+
+def face_zphi(face: int, x, y):
+    """(z, phi) of continuous face coordinates (x, y) in [0, 1]^2 (equal-area)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    jr = _JRLL[face] - x - y
+    z = np.where(jr < 1, 1 - jr**2 / 3, np.where(jr > 3, (4 - jr) ** 2 / 3 - 1, (2 - jr) * 2 / 3))
+    nr = np.where(jr < 1, jr, np.where(jr > 3, 4 - jr, 1.0))
+    phi = (np.pi / 4) * (_JPLL[face] * nr + x - y) / np.where(nr > 0, nr, 1.0)
+    return z, phi
+
+
+def pixel_nodes(face: int, ix: int, iy: int, nside: int, G: int = 6):
+    """Quadrature (z, phi, w) of one pixel, weights summing to 1: the pixel is
+    split along its centre line x + y = const into two triangles, each
+    integrated in collapsed coordinates toward its apex (G x G Gauss-Legendre)."""
+    t, wt = np.polynomial.legendre.leggauss(G)
+    u, wu = (t + 1) / 2, wt / 2
+    h0 = 1.0 / nside
+    sc, dc = (ix + iy + 1) * h0, (ix - iy) * h0
+    zs, ps, ws = [], [], []
+    for sgn in (1.0, -1.0):
+        s = np.repeat(sc + sgn * h0 * u, G)
+        h = np.repeat(h0 * (1 - u), G)
+        d = dc + h * np.tile(t, G)
+        w = np.repeat(wu * (1 - u), G) * np.tile(wt, G) / 2
+        z, phi = face_zphi(face, (s + d) / 2, (s - d) / 2)
+        zs.append(z)
+        ps.append(phi)
+        ws.append(w)
+    return np.concatenate(zs), np.concatenate(ps), np.concatenate(ws)
+
+
+def wigner_d(l: int, mp: int, m: int, beta) -> np.ndarray:
+    """d^l_{mp, m}(beta) by Wigner's explicit sum (small l only)."""
+    f = math.factorial
+    c, s = np.cos(np.asarray(beta) / 2), np.sin(np.asarray(beta) / 2)
+    out = np.zeros_like(c)
+    pre = math.sqrt(f(l + mp) * f(l - mp) * f(l + m) * f(l - m))
+    for k in range(max(0, m - mp), min(l + m, l - mp) + 1):
+        den = f(l + m - k) * f(k) * f(mp - m + k) * f(l - mp - k)
+        out = out + (-1) ** (mp - m + k) * pre / den * c ** (2 * l + m - mp - 2 * k) * s ** (mp - m + 2 * k)
+    return out
+
+
+def pixwin_definition(nside: int, lmax: int, spin: int = 0, G: int = 10) -> np.ndarray:
+    """w^s_l from the definition, every pixel (small nside and lmax only)."""
+    W = np.zeros(lmax + 1)
+    for f in range(12):
+        for ix in range(nside):
+            for iy in range(nside):
+                z, phi, w = pixel_nodes(f, ix, iy, nside, G)
+                th = np.arccos(np.clip(z, -1, 1))
+                for l in range(abs(spin), lmax + 1):
+                    W[l] += sum(abs(np.sum(w * wigner_d(l, m, -spin, th) * np.exp(1j * m * phi))) ** 2
+                                for m in range(-l, l + 1))
+    return np.sqrt(W / (12 * nside * nside))
+
+
+def _pair_power(z, phi, w, lmax: int, spin: int):
+    st = np.sqrt(np.maximum(0.0, (1 - z) * (1 + z)))
+    n = np.stack([st * np.cos(phi), st * np.sin(phi), z], -1)
+    eth = np.stack([z * np.cos(phi), z * np.sin(phi), -st], -1)
+    eph = np.stack([-np.sin(phi), np.cos(phi), np.zeros_like(phi)], -1)
+    D = n[None, :, :] - n[:, None, :]  # [a, b] = n_b - n_a
+    q = 0.5 * np.sum(D * D, axis=-1)   # 1 - cos beta
+    beta = 2 * np.arcsin(np.minimum(1.0, np.sqrt(2 * q) / 2))
+    x = np.cos(beta)
+    t1 = D + q[..., None] * n[:, None, :]   # at a, toward b
+    t2 = D - q[..., None] * n[None, :, :]   # at b, away from a
+    c1 = np.einsum("abk,ak->ab", t1, eth)
+    s1 = np.einsum("abk,ak->ab", t1, eph)
+    c2 = np.einsum("abk,bk->ab", t2, eth)
+    s2 = np.einsum("abk,bk->ab", t2, eph)
+    re, im = c2 * c1 + s2 * s1, s2 * c1 - c2 * s1
+    nn = re * re + im * im
+    safe = np.where(nn > 0, nn, 1.0)
+    if spin == 2:
+        cs = np.where(nn > 0, (re * re - im * im) / safe, 1.0)
+    elif spin == 1:
+        cs = np.where(nn > 0, re / np.sqrt(safe), 1.0)
+    else:
+        cs = np.ones_like(x)
+    wp = np.outer(w, w) * cs
+    out = np.zeros(lmax + 1)
+    s2_ = spin * spin
+    if spin == 0:
+        p0, p1 = np.ones_like(x), x
+        out[0] = np.sum(wp)
+        if lmax >= 1:
+            out[1] = np.sum(wp * x)
+        for l in range(2, lmax + 1):
+            p0, p1 = p1, ((2 * l - 1) * x * p1 - (l - 1) * p0) / l
+            out[l] = np.sum(wp * p1)
+        return out
+    d0, d1 = np.zeros_like(x), ((1 + x) / 2) ** spin
+    if lmax >= spin:
+        out[spin] = np.sum(wp * d1)
+    for l in range(spin + 1, lmax + 1):
+        lm = l - 1
+        d0, d1 = d1, ((2 * l - 1) * (lm * (lm + 1) * x - s2_) * d1 - l * (lm * lm - s2_) * d0) / (lm * (l * l - s2_))
+        out[l] = np.sum(wp * d1)
+    return out
+
+
+def pixwin_pairs(nside: int, lmax: int, spin: int = 0, G: int = 6, reduced: bool = True) -> np.ndarray:
+    """w^s_l in the pair form, exact per pair.  reduced: only the pixels the
+    device computes (face 0 with ix <= iy, ix + iy >= nside - 1, weight 8 or 16,
+    and one pixel per belt ring nside + 1 .. 2 nside, weight 8 nside or 4 nside
+    on the equator); otherwise every pixel."""
+    W = np.zeros(lmax + 1)
+    if reduced:
+        items = [(0, ix, iy, 8.0 if ix == iy else 16.0)
+                 for iy in range(nside) for ix in range(iy + 1) if ix + iy >= nside - 1]
+        items += [(4, nside - 1, 2 * nside - r, 4.0 * nside if r == 2 * nside else 8.0 * nside)
+                  for r in range(nside + 1, 2 * nside + 1)]
+    else:
+        items = [(f, ix, iy, 1.0) for f in range(12) for ix in range(nside) for iy in range(nside)]
+    for f, ix, iy, mult in items:
+        W += mult * _pair_power(*pixel_nodes(f, ix, iy, nside, G), lmax, spin)
+    return np.sqrt(np.maximum(W / (12 * nside * nside), 0.0))
+
+
+# --------------------------------------------------------------------------
+# Multi-plane convergence (glass/lensing.py:430-673), restated with numpy
+# arrays exactly as the reference computes it.
+
+
+class MultiPlaneConvergence:
+    """glass/lensing.py:430-586."""
+
+    def __init__(self, cosmo) -> None:
+        self.cosmo = cosmo
+        self.z2 = 0.0
+        self.z3 = 0.0
+        self.x3 = 0.0
+        self.w3 = 0.0
+        self.r23 = 1.0
+        self.delta3 = None
+        self.kappa2 = None
+        self.kappa3 = None
+
+    def add_window(self, delta, w) -> None:
+        """glass/lensing.py:488-508."""
+        if self.delta3 is None:
+            self.delta3 = np.asarray(0.0)
+        zsrc = w.zeff
+        lens_weight = float(np.trapezoid(w.wa, w.za) / np.interp(zsrc, w.za, w.wa))
+        self.add_plane(delta, zsrc, lens_weight)
+
+    def add_plane(self, delta, zsrc, wlens: float = 1.0) -> None:
+        """glass/lensing.py:510-586."""
+        if self.delta3 is None:
+            self.delta3 = np.asarray(0.0)
+        if zsrc <= self.z3:
+            raise ValueError("source redshift must be increasing")
+        delta2, self.delta3 = self.delta3, delta
+        z1, self.z2, self.z3 = self.z2, self.z3, zsrc
+        w2, self.w3 = self.w3, wlens
+        x2, self.x3 = (self.x3,
+                       self.cosmo.transverse_comoving_distance(self.z3) / self.cosmo.hubble_distance)
+        r12 = self.r23
+        r13, self.r23 = self.cosmo.transverse_comoving_distance(
+            [z1, self.z2], self.z3) / (self.cosmo.hubble_distance * self.x3)
+        t = r13 / r12
+        f = 3 * self.cosmo.Omega_m0 / 2
+        f *= x2 * self.r23
+        f *= (1 + self.z2) / self.cosmo.H_over_H0(self.z2)
+        f *= w2
+        if self.kappa2 is None:
+            self.kappa2 = np.zeros_like(delta)
+            self.kappa3 = np.zeros_like(delta)
+        self.kappa2, self.kappa3 = self.kappa3, self.kappa2
+        t = np.asarray(t)
+        self.kappa3 *= 1 - t
+        self.kappa3 += t * self.kappa2
+        self.kappa3 += np.asarray(f) * delta2
+
+    @property
+    def kappa(self):
+        return self.kappa3
+
+
+def multi_plane_matrix(shells, cosmo) -> np.ndarray:
+    """glass/lensing.py:589-616."""
+    mpc = MultiPlaneConvergence(cosmo)
+    wmat = np.eye(len(shells))
+    for i, w in enumerate(shells):
+        mpc.add_window(wmat[i, :].copy(), w)
+        wmat[i, :] = mpc.kappa
+    return wmat
+
+
+def multi_plane_weights(weights, shells, cosmo) -> np.ndarray:
+    """glass/lensing.py:636-673."""
+    shape = weights.shape
+    if not shape or shape[0] != len(shells):
+        raise ValueError("shape mismatch between weights and shells")
+    weights = weights / np.sum(weights, axis=0)
+    return np.matmul(multi_plane_matrix(shells, cosmo).T, weights)

@@ -78,6 +78,14 @@ def test_shear_batch_rejects_bad_shape():
         eng.run(torch.zeros((3, 12 * 64), dtype=torch.float32, device="cuda"))
 
 
+def _trelerr(got, ref):
+    """max |got - ref| / RMS(ref) and RMS(got - ref) / RMS(ref), on the device."""
+    got, ref = got.double(), ref.double()
+    rms = torch.sqrt(torch.mean(ref * ref))
+    d = got - ref
+    return float(torch.max(torch.abs(d)) / rms), float(torch.sqrt(torch.mean(d * d)) / rms)
+
+
 def _check_shells(nside, lmax, S, shells):
     gls = shell_spectra(S, lmax, amp=lambda i: 1.0 + 3.0 * i / max(S - 1, 1),
                         rho=(0.6, 0.3, 0.1), scale=1500.0)
@@ -89,15 +97,16 @@ def _check_shells(nside, lmax, S, shells):
     out = []
     for s in shells:
         ref = eng.fp64_reference_map(s)
-        e_map = relerr(eng.maps[s].double().cpu().numpy(), ref.cpu().numpy())
+        e_map, r_map = _trelerr(eng.maps[s], ref)
         # FP64 shear of the FP32 map (same input, so only the shear's own error counts)
         kappa = eng.maps[s].double()
         alm = hp.map2alm(kappa, lmax=lmax, pol=False, precision="fp64")
         alm = hp.almxfl(alm, fl)
         q, u = hp.alm2map_spin([alm, torch.zeros_like(alm)], nside, 2, lmax)
-        e_q = relerr(eng.gamma[s, 0].cpu().numpy(), q.cpu().numpy())
-        e_u = relerr(eng.gamma[s, 1].cpu().numpy(), u.cpu().numpy())
-        print(f"shell {s}: map {e_map:.2e} gamma1 {e_q:.2e} gamma2 {e_u:.2e}", flush=True)
+        e_q, _ = _trelerr(eng.gamma[s, 0], q)
+        e_u, _ = _trelerr(eng.gamma[s, 1], u)
+        print(f"shell {s}: map max {e_map:.2e} rms {r_map:.2e}  gamma1 {e_q:.2e} "
+              f"gamma2 {e_u:.2e}", flush=True)
         out.append((e_map, e_q, e_u))
     return out
 
@@ -130,9 +139,12 @@ def test_lognormal_shells_with_shear_small():
 def test_lognormal_shells_with_shear_64_shells_nside1024():
     """64 correlated lognormal shells at nside 1024, lmax 2047, ncorr 3 (spectra
     amplitude x1 -> x4, neighbour correlation 0.6/0.3/0.1), shear for all 64;
-    per-shell error against the FP64 path for three shells."""
-    for e_map, e_q, e_u in _check_shells(1024, 2047, 64, [0, 31, 63]):
-        assert e_map < 1e-5 and e_q < 1e-5 and e_u < 1e-5
+    every shell's map and both shear components within 1e-5 of the FP64 path
+    (max |error| / RMS, each printed)."""
+    errs = _check_shells(1024, 2047, 64, range(64))
+    worst = np.max(np.array(errs), axis=0)
+    print(f"worst of 64 shells: map {worst[0]:.2e} gamma1 {worst[1]:.2e} gamma2 {worst[2]:.2e}")
+    assert np.all(worst < 1e-5), worst
 
 
 def test_from_convergence_fp32_batch_agrees_with_module_function():
@@ -295,14 +307,18 @@ def test_wave_pair_legendre_matches_single_wave(nside, lmax, S):
     (tiles = 4: one wave runs the recurrence, its partner the MFMAs) issues
     the same MFMAs in the same order as the single-wave form with two tiles,
     so the maps agree bit for bit."""
-    from glass_amd import autotune
+    from glass_amd import _lib, autotune
 
     print(f"wave pairs nside={nside} lmax={lmax} S={S}", flush=True)
     gls = shell_spectra(S, lmax, rho=(0.6, 0.3, 0.1))
     eng = LognormalShells(gls, nside, ncorr=3, seed=11, lmax=lmax)
     saved = autotune.current(eng.plan)
+    saved_flush = eng.plan.get_tuning(_lib.TUNE_F16_FLUSH)
     out = {}
     try:
+        # the wave-pair form accumulates each l-sum in one chain: compare it
+        # with the single-wave form in the same accumulation form
+        eng.plan.set_tuning(_lib.TUNE_F16_FLUSH, -1)
         for nt in (2, 4):
             autotune.apply(eng.plan, {"path": 2, "nt": nt})
             eng.run()
@@ -310,6 +326,7 @@ def test_wave_pair_legendre_matches_single_wave(nside, lmax, S):
             out[nt] = eng.maps.clone()
     finally:
         autotune.apply(eng.plan, saved)
+        eng.plan.set_tuning(_lib.TUNE_F16_FLUSH, saved_flush)
     assert torch.equal(out[2], out[4])
     ref = eng.fp64_reference_map(S - 1).cpu().numpy()
     assert relerr(out[4][S - 1].cpu().numpy(), ref) < 1e-5

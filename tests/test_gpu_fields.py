@@ -224,8 +224,6 @@ def test_from_convergence_chain():
         assert np.max(np.abs(got - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-10
     g1, g2 = glass_amd.shear_from_convergence(kappa, lmax, discretized=False)
     assert np.max(np.abs(g1 + 1j * g2 - gamma)) / np.sqrt(np.mean(np.abs(gamma) ** 2)) < 1e-10
-    with pytest.raises(NotImplementedError, match="pixwin_ratio"):
-        glass_amd.from_convergence(kappa, lmax, shear=True)
 
 
 @pytest.mark.parametrize("shell,seed", [(0, 7), (5, 2**40 + 3)])

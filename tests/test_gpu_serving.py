@@ -69,3 +69,31 @@ def test_budget_admission_on_device():
     assert sched.peak <= need * 1.5
     for i in range(3):
         assert torch.isfinite(ex.results[i]).all()
+
+
+def test_c5_geometries_alone_mixed_and_fp64():
+    """One request per serving geometry (nside 256 .. 4096, lmax = 2 nside,
+    ncorr 3, a few shells each), all arriving together through the device
+    executor: every shell equals the same request generated alone by the
+    direct batch engine to 1e-6, and the FP64 path to 1e-5 (lmax 8192 at
+    nside 4096 included)."""
+    reqs = [SimRequest(i, 0.0, ns, n, 2 * ns, ncorr=3, seed=400 + i)
+            for i, (ns, n) in enumerate(((256, 5), (512, 4), (1024, 4), (2048, 3), (4096, 3)))]
+    ex = DeviceExecutor()
+    sched = run(reqs, ex, 240e9, keep_results=True)
+    assert sorted(c.rid for c in sched.completed) == list(range(len(reqs)))
+    worst = {}
+    for r in reqs:
+        print(f"C5 request nside={r.nside} lmax={r.lmax} shells={r.nshell}", flush=True)
+        mixed = ex.results[r.rid]
+        eng = LognormalShells(r.spectra(), r.nside, ncorr=r.ncorr, shift=r.shift, seed=r.seed,
+                              lmax=r.lmax)
+        eng.run()
+        torch.cuda.synchronize()
+        e_alone = max(rel(mixed[s], eng.maps[s]) for s in range(r.nshell))
+        e_fp64 = max(rel(mixed[s], eng.fp64_reference_map(s)) for s in range(r.nshell))
+        worst[r.nside] = (e_alone, e_fp64)
+        print(f"  alone vs mixed {e_alone:.2e}, vs FP64 {e_fp64:.2e}", flush=True)
+        del eng
+        torch.cuda.empty_cache()
+        assert e_alone < 1e-6 and e_fp64 < 1e-5, (r.nside, e_alone, e_fp64)

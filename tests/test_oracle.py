@@ -469,3 +469,134 @@ def test_generate_window_implies_the_banded_covariance(nfields, ncorr):
         for s in range(nfields):
             if s > t or s < t - ncorr:
                 assert np.all(L[:, t, s] == 0.0), (t, s)
+
+
+# ---- pixel windows -------------------------------------------------------
+
+
+@pytest.mark.parametrize("nside", [1, 2, 4])
+def test_face_geometry_gives_the_pixel_centres(nside):
+    """Face coordinates at pixel centres ((ix + 1/2) / nside, (iy + 1/2) / nside)
+    land on the HEALPix pixel centres of the ring geometry, one for one."""
+    th, ph = O.pixel_angles(nside)
+    ref = np.stack([np.cos(th), np.mod(ph, 2 * np.pi)], -1)
+    got = []
+    for f in range(12):
+        for ix in range(nside):
+            for iy in range(nside):
+                z, p = O.face_zphi(f, (ix + 0.5) / nside, (iy + 0.5) / nside)
+                got.append((float(z), float(np.mod(p, 2 * np.pi))))
+    got = np.array(sorted(got))
+    ref = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, atol=1e-12)
+
+
+def test_pixel_quadrature_is_area_weighted():
+    """The nodes of a pixel integrate z and z^2 like the exact area average of
+    its ring range, for a belt pixel (z linear in x + y) and a polar one."""
+    nside = 4
+    for f, ix, iy in ((4, 1, 2), (0, 3, 3), (0, 2, 3)):
+        z, _, w = O.pixel_nodes(f, ix, iy, nside, G=6)
+        assert abs(w.sum() - 1) < 1e-14
+        # brute force: a fine uniform grid in the face coordinates
+        g = (np.arange(400) + 0.5) / 400
+        X, Y = np.meshgrid((ix + g) / nside, (iy + g) / nside)
+        zb, _ = O.face_zphi(f, X, Y)
+        assert abs(np.sum(w * z) - zb.mean()) < 1e-6
+        assert abs(np.sum(w * z * z) - (zb**2).mean()) < 1e-6
+
+
+def test_wigner_d_matches_the_harmonics():
+    """sqrt((2l+1)/4pi) |d^l_{m,-s}| equals |sY_lm| of the oracle's Legendre
+    (s = 0, pinned to scipy) and spin-weighted (s = 2, pinned by sympy) tables."""
+    theta = 0.7
+    for m in range(0, 5):
+        lam = O.legendre(m, 8, theta)
+        for l in range(m, 9):
+            d = O.wigner_d(l, m, 0, np.array([theta]))[0]
+            assert abs(abs(d) * math.sqrt((2 * l + 1) / (4 * math.pi)) - abs(lam[l - m])) < 1e-13
+        sp, _ = O.spin_legendre(2, m, 8, theta)
+        l0 = max(m, 2)
+        for l in range(l0, 9):
+            d = O.wigner_d(l, m, -2, np.array([theta]))[0]
+            assert abs(abs(d) * math.sqrt((2 * l + 1) / (4 * math.pi)) - abs(sp[l - l0])) < 1e-13
+
+
+@pytest.mark.parametrize("nside,spin", [(1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2)])
+def test_pixwin_pair_form_matches_definition(nside, spin):
+    """The pair form (what the device computes) against the definition evaluated
+    directly from the harmonics, every pixel: pins the addition theorem and the
+    frame-rotation angle of the spin-2 window; both quadratures converged to
+    ~1e-7 at these (largest) pixel sizes."""
+    lmax = 4 * nside
+    d = O.pixwin_definition(nside, lmax, spin, G=10)
+    p = O.pixwin_pairs(nside, lmax, spin)
+    assert np.max(np.abs(d - p)) < 3e-7
+    if spin:
+        assert np.all(p[:spin] == 0)
+
+
+@pytest.mark.parametrize("nside", [2, 4])
+def test_pixwin_symmetry_reduction(nside):
+    """The reduced pixel set with its multiplicities equals the sum over every pixel."""
+    for spin in (0, 2):
+        full = O.pixwin_pairs(nside, 4 * nside, spin, reduced=False)
+        red = O.pixwin_pairs(nside, 4 * nside, spin)
+        assert np.max(np.abs(full - red)) < 1e-13
+
+
+def test_pixwin_limits():
+    """w_0 = 1; the windows decrease with l up to 4 nside and stay in (0, 1]."""
+    for nside in (4, 8):
+        pw0 = O.pixwin_pairs(nside, 4 * nside, 0)
+        pw2 = O.pixwin_pairs(nside, 4 * nside, 2)
+        assert abs(pw0[0] - 1) < 1e-14
+        assert np.all(np.diff(pw0) < 0) and pw0[-1] > 0
+        assert np.all(np.diff(pw2[2:]) < 0) and pw2[-1] > 0 and pw2[2] < 1
+
+
+# ---- multi-plane convergence ----------------------------------------------
+
+
+def test_multi_plane_matrix_identity(cosmo, shells):
+    """tests/core/test_lensing.py:64-86: the matrix is lower triangular and
+    maps random mass planes to the convergence sequence."""
+    mat = O.multi_plane_matrix(shells, cosmo)
+    assert np.array_equal(mat, np.tril(mat))
+    rng = np.random.default_rng(1)
+    deltas = rng.random((len(shells), 10))
+    conv = O.MultiPlaneConvergence(cosmo)
+    kappas = []
+    for i, w in enumerate(shells):
+        conv.add_window(deltas[i], w)
+        kappas.append(conv.kappa.copy())
+    assert np.allclose(mat @ deltas, np.stack(kappas), rtol=1e-12, atol=1e-15)
+    # the first source plane sees no mass in front of it
+    assert np.all(mat[0] == 0)
+
+
+def test_multi_plane_weights_identity(cosmo, shells):
+    """tests/core/test_lensing.py:89-120."""
+    w_out = O.multi_plane_weights(np.eye(len(shells)), shells, cosmo)
+    assert np.array_equal(w_out, np.triu(w_out, k=1))
+    rng = np.random.default_rng(2)
+    deltas = rng.random((len(shells), 10))
+    weights = rng.random((len(shells), 3))
+    conv = O.MultiPlaneConvergence(cosmo)
+    kappa = 0
+    for i, w in enumerate(shells):
+        conv.add_window(deltas[i], w)
+        kappa = kappa + weights[i][..., None] * conv.kappa
+    kappa /= np.sum(weights, axis=0)[..., None]
+    wmat = O.multi_plane_weights(weights, shells, cosmo)
+    assert np.allclose(np.einsum("ij,ik", wmat, deltas), kappa, rtol=1e-12)
+    with pytest.raises(ValueError, match="shape mismatch"):
+        O.multi_plane_weights(np.ones(3), shells, cosmo)
+
+
+def test_multi_plane_source_order(cosmo):
+    conv = O.MultiPlaneConvergence(cosmo)
+    conv.add_plane(np.zeros(4), 1.0)
+    with pytest.raises(ValueError, match="source redshift must be increasing"):
+        conv.add_plane(np.zeros(4), 1.0)
