@@ -90,6 +90,11 @@ _SIGNATURES = {
     "ga_pixwin": [_i32, _i32, _i32, _vp, _vp, _vp, _sz, _vp],
     "ga_multi_plane_update": [_i64, _i32, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _vp,
                               _vp],
+    "ga_points_workspace_size": [_i64, ctypes.POINTER(_sz)],
+    "ga_points_counts": [_i64, _i32, _vp, _i32, ctypes.c_double, _i32, ctypes.c_double, _vp, _u64,
+                         _i64, _vp, _vp, _vp, _vp, _sz, _vp],
+    "ga_points_scan": [_i64, _vp, _vp, _vp, _sz, _vp],
+    "ga_points_positions": [_i32, _vp, _u64, _i64, _vp, _vp, _vp, _sz, _vp],
 }
 
 _lib = None

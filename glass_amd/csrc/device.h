@@ -119,6 +119,30 @@ __device__ __forceinline__ void to_scaled(double mant, long long E, double &p, i
   p = ldexp(mant, rem);
 }
 
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011), counter-based
+struct u32x4 {
+  uint32_t v[4];
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c.v[0];
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.v[2];
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    c = u32x4{{hi1 ^ c.v[1] ^ k0, lo1, hi0 ^ c.v[3] ^ k1, lo0}};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// a double in [0, 1) with 53 random bits from two words (27 + 26 bits)
+__device__ __forceinline__ double uniform53(uint32_t a, uint32_t b) {
+  return (double(a >> 5) * 67108864.0 + double(b >> 6)) * 0x1p-53;
+}
+
 // full 64-bit lane masks
 __device__ __forceinline__ bool wave_any(bool v) { return __any(v); }
 __device__ __forceinline__ bool wave_all(bool v) { return __all(v); }

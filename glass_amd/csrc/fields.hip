@@ -31,25 +31,6 @@ __device__ __forceinline__ int glass_l(int64_t g) {
   return l;
 }
 
-// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw 2011), counter-based
-struct u32x4 {
-  uint32_t v[4];
-};
-
-__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint64_t p0 = uint64_t(0xD2511F53u) * c.v[0];
-    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.v[2];
-    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
-    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
-    c = u32x4{{hi1 ^ c.v[1] ^ k0, lo1, hi0 ^ c.v[3] ^ k1, lo0}};
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
-  return c;
-}
-
 // standard complex normal pair (re, im) for GLASS-order index g of shell t:
 // Box-Muller on two Philox words, u1 = (w0 + 1/2) 2^-32 in (0, 1] (tail to
 // sqrt(-2 ln 2^-33) = 6.76 sigma) and u2 = (w1 >> 8) 2^-24 in [0, 1), with the

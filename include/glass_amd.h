@@ -272,6 +272,37 @@ int ga_pixwin(int32_t nside, int32_t lmax, int32_t spin, double *pw0, double *pw
 int ga_multi_plane_update(int64_t n, int32_t real_bytes, const void *kappa1, const void *kappa2,
                           const void *delta2, double t, double f, void *kappa3, void *stream);
 
+/* Points from a density contrast (glass/points.py:443-540), one population
+ * per call on npix-pixel RING maps.  Workspace: ga_points_workspace_size.
+ * ga_points_counts: the expected count per pixel (glass/points.py:214-316)
+ *   lam = max(0, (b(delta) [- mean] + 1) * scale [* vis]),
+ * scale = ARCMIN2_SPHERE / npix * ngal, b(delta) by model 0 = delta (bias
+ * None, or a bias model applied by the caller), 1 = linear (bias * delta,
+ * glass/points.py:115-134), 2 = log-linear (expm1(bias * log1p(delta)),
+ * :137-160); delta float32 or float64 (real_bytes), vis float64 or NULL; the
+ * mean is subtracted when remove_monopole.  Writes lam (float64, or NULL)
+ * and/or counts (int64, or NULL): Poisson counts from the counter-based
+ * Philox4x32-10 generator keyed by (seed, population, pixel), with the total
+ * count in *total (device int64) and the block offsets kept in the workspace
+ * for ga_points_positions.
+ * ga_points_scan: the same offsets and total for counts drawn elsewhere (the
+ *   host generator of glass/points.py:319-348).
+ * ga_points_positions: every point of the counts placed uniformly in area
+ *   inside its pixel (lon in [0, 360), lat, degrees, float64, total points),
+ *   uniforms keyed by (seed, population, point); the workspace must hold the
+ *   offsets of the same counts.  Stands in for healpix.randang as called at
+ *   glass/points.py:425-429 (glass/healpix.py:398-432). */
+int ga_points_workspace_size(int64_t npix, size_t *bytes);
+int ga_points_counts(int64_t npix, int32_t real_bytes, const void *delta, int32_t model,
+                     double bias, int32_t remove_monopole, double scale, const double *vis,
+                     uint64_t seed, int64_t population, double *lam, int64_t *counts,
+                     int64_t *total, void *workspace, size_t ws_bytes, void *stream);
+int ga_points_scan(int64_t npix, const int64_t *counts, int64_t *total, void *workspace,
+                   size_t ws_bytes, void *stream);
+int ga_points_positions(int32_t nside, const int64_t *counts, uint64_t seed, int64_t population,
+                        double *lon, double *lat, void *workspace, size_t ws_bytes,
+                        void *stream);
+
 /* Pointwise transforms on a float64 map (n pixels), out may equal x.
  * glass/grf/_transformations.py. */
 int ga_transform_f64(int64_t n, const double *x, const ga_epilogue *epi_host, double *out,

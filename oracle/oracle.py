@@ -682,3 +682,186 @@ def multi_plane_weights(weights, shells, cosmo) -> np.ndarray:
         raise ValueError("shape mismatch between weights and shells")
     weights = weights / np.sum(weights, axis=0)
     return np.matmul(multi_plane_matrix(shells, cosmo).T, weights)
+
+
+# --------------------------------------------------------------------------
+# Points from a density contrast (glass/points.py:115-540), and the device
+# generator's counts and positions restated (no counterpart in the reference,
+# whose positions come from the healpix package's randang).
+
+ARCMIN2_SPHERE = 60**6 // 100 / math.pi
+
+
+def expected_counts(delta, ngal, bias=None, vis=None, model="linear", remove_monopole=False):
+    """glass/points.py:214-348 up to the Poisson draw: the clipped expected
+    number of points per pixel, with the reference's array operations."""
+    if bias is None:
+        n = np.array(delta, dtype=np.float64, copy=True)
+    elif model == "linear":
+        n = bias * np.asarray(delta, dtype=np.float64)
+    else:
+        n = np.expm1(np.log1p(np.asarray(delta, dtype=np.float64)) * bias)
+    if remove_monopole:
+        n = n - np.mean(n, keepdims=True)
+    n = n + 1
+    n *= ARCMIN2_SPHERE / n.size * ngal
+    if vis is not None:
+        n *= vis
+    return np.clip(n, 0.0, None)
+
+
+def uniform53(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    return ((a >> np.uint32(5)).astype(np.float64) * 67108864.0
+            + (b >> np.uint32(6)).astype(np.float64)) * 2.0**-53
+
+
+def _philox_words(lo, hi, w2, w3, seed):
+    ctr = np.stack([lo, hi, w2, w3], axis=-1).astype(np.uint32)
+    key = np.broadcast_to(np.array([seed & 0xFFFFFFFF, seed >> 32], dtype=np.uint32),
+                          ctr.shape[:-1] + (2,))
+    return philox4x32_10(ctr, key)
+
+
+def device_poisson(lam: np.ndarray, seed: int, population: int) -> np.ndarray:
+    """Counts of the device generator: lambda cut into parts of at most 32,
+    each inverted on the CDF with one 53-bit uniform from Philox4x32-10 at
+    counter (pixel lo, pixel hi, population, 0x40000000 + part)."""
+    lam = np.asarray(lam, dtype=np.float64).copy()
+    pix = np.arange(lam.size, dtype=np.uint64)
+    out = np.zeros(lam.size, dtype=np.int64)
+    part = 0
+    while np.any(lam > 0):
+        act = np.nonzero(lam > 0)[0]
+        lpart = np.minimum(lam[act], 32.0)
+        lam[act] = lam[act] - lpart
+        w = _philox_words(pix[act] & np.uint64(0xFFFFFFFF), pix[act] >> np.uint64(32),
+                          np.full(act.size, population, np.uint64),
+                          np.full(act.size, 0x40000000 + part, np.uint64), seed)
+        part += 1
+        u = uniform53(w[:, 0], w[:, 1])
+        p = np.exp(-lpart)
+        cdf = p.copy()
+        k = np.zeros(act.size, dtype=np.int64)
+        run = u > cdf
+        while np.any(run):
+            k[run] += 1
+            p[run] = p[run] * (lpart[run] / k[run])
+            cdf[run] = cdf[run] + p[run]
+            run = run & (u > cdf) & (k < 2048)
+        out[act] += k
+    return out
+
+
+_JRLL_A = np.array(_JRLL)
+_JPLL_A = np.array(_JPLL)
+
+
+def ring_to_xyf(nside: int, pix) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """RING pixel -> (face, ix, iy) of the HEALPix face grid (Gorski et al. 2005)."""
+    pix = np.asarray(pix, dtype=np.int64)
+    N = int(nside)
+    ncap, npix, nl2 = 2 * N * (N - 1), 12 * N * N, 2 * N
+    face = np.zeros(pix.shape, np.int64)
+    iring = np.zeros(pix.shape, np.int64)
+    iphi = np.zeros(pix.shape, np.int64)
+    kshift = np.zeros(pix.shape, np.int64)
+    nr = np.zeros(pix.shape, np.int64)
+    n = pix < ncap
+    if np.any(n):
+        p = pix[n]
+        ir = (1 + np.floor(np.sqrt(1.0 + 2 * p)).astype(np.int64)) >> 1
+        ir = np.where(2 * ir * (ir - 1) > p, ir - 1, ir)
+        ir = np.where(2 * (ir + 1) * ir <= p, ir + 1, ir)
+        ip = (p + 1) - 2 * ir * (ir - 1)
+        iring[n], iphi[n], nr[n] = ir, ip, ir
+        face[n] = (ip - 1) // ir
+    e = (pix >= ncap) & (pix < npix - ncap)
+    if np.any(e):
+        ip0 = pix[e] - ncap
+        tmp = ip0 // (4 * N)
+        ir = tmp + N
+        ip = ip0 - tmp * 4 * N + 1
+        ks = (ir + N) & 1
+        ire = ir - N + 1
+        irm = nl2 + 2 - ire
+        ifm = (ip - ire // 2 + N - 1) // N
+        ifp = (ip - irm // 2 + N - 1) // N
+        f = np.where(ifp == ifm, ifp | 4, np.where(ifp < ifm, ifp, ifm + 8))
+        iring[e], iphi[e], kshift[e], nr[e], face[e] = ir, ip, ks, N, f
+    s = pix >= npix - ncap
+    if np.any(s):
+        ip0 = npix - pix[s]
+        ir = (1 + np.floor(np.sqrt(2.0 * ip0 - 1)).astype(np.int64)) >> 1
+        ir = np.where(2 * ir * (ir - 1) >= ip0, ir - 1, ir)
+        ir = np.where(2 * (ir + 1) * ir < ip0, ir + 1, ir)
+        ip = 4 * ir + 1 - (ip0 - 2 * ir * (ir - 1))
+        iring[s], iphi[s], nr[s] = 2 * nl2 - ir, ip, ir
+        face[s] = 8 + (ip - 1) // ir
+    irt = iring - _JRLL_A[face] * N + 1
+    ipt = 2 * iphi - _JPLL_A[face] * nr - kshift - 1
+    ipt = np.where(ipt >= nl2, ipt - 8 * N, ipt)
+    return face, (ipt - irt) >> 1, (-ipt - irt) >> 1
+
+
+def face_lonlat(face, x, y) -> tuple[np.ndarray, np.ndarray]:
+    """(lon in [0, 360), lat) in degrees of face coordinates (x, y)."""
+    face = np.asarray(face)
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    jr = _JRLL_A[face] - x - y
+    polar_n, polar_s = jr < 1, jr > 3
+    nr = np.where(polar_n, jr, np.where(polar_s, 4 - jr, 1.0))
+    t = nr * nr / 3
+    z = np.where(polar_n, 1 - t, np.where(polar_s, t - 1, (2 - jr) * (2 / 3)))
+    st = np.where(polar_n | polar_s, np.sqrt(t * (2 - t)), np.sqrt((1 - z) * (1 + z)))
+    q = np.where(nr > 0, (_JPLL_A[face] * nr + x - y) / np.where(nr > 0, nr, 1.0), 0.0)
+    q = q - 8.0 * np.floor(q / 8.0)
+    return 45.0 * q, np.degrees(np.arctan2(z, st))
+
+
+def device_positions(nside: int, counts: np.ndarray, seed: int, population: int):
+    """Positions of the device generator: point q of pixel p at face
+    coordinates ((ix + u) / nside, (iy + v) / nside), u, v 53-bit uniforms from
+    Philox4x32-10 at counter (q lo, q hi, population, 0x80000000)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    ipix = np.repeat(np.arange(counts.size), counts)
+    q = np.arange(ipix.size, dtype=np.uint64)
+    w = _philox_words(q & np.uint64(0xFFFFFFFF), q >> np.uint64(32),
+                      np.full(q.size, population, np.uint64),
+                      np.full(q.size, 0x80000000, np.uint64), seed)
+    u, v = uniform53(w[:, 0], w[:, 1]), uniform53(w[:, 2], w[:, 3])
+    f, ix, iy = ring_to_xyf(nside, ipix)
+    lon, lat = face_lonlat(f, (ix + u) / nside, (iy + v) / nside)
+    return lon, lat, ipix
+
+
+def ang2pix_ring(nside: int, theta, phi) -> np.ndarray:
+    """RING pixel containing (theta, phi) (Gorski et al. 2005, the ring scheme)."""
+    N = int(nside)
+    z = np.cos(np.asarray(theta, dtype=np.float64))
+    za = np.abs(z)
+    tt = np.mod(np.asarray(phi, dtype=np.float64), 2 * np.pi) / (np.pi / 2)  # in [0, 4)
+    out = np.zeros(z.shape, np.int64)
+    eq = za <= 2 / 3
+    if np.any(eq):
+        t1 = N * (0.5 + tt[eq])
+        t2 = N * z[eq] * 0.75
+        jp = np.floor(t1 - t2).astype(np.int64)
+        jm = np.floor(t1 + t2).astype(np.int64)
+        ir = N + 1 + jp - jm
+        ks = 1 - (ir & 1)
+        ip = (jp + jm - N + ks + 1) // 2
+        ip = np.mod(ip, 4 * N)
+        out[eq] = 2 * N * (N - 1) + (ir - 1) * 4 * N + ip
+    pc = ~eq
+    if np.any(pc):
+        ttp = tt[pc]
+        tp = ttp - np.floor(ttp)
+        tmp = N * np.sqrt(3 * (1 - za[pc]))
+        jp = np.floor(tp * tmp).astype(np.int64)
+        jm = np.floor((1 - tp) * tmp).astype(np.int64)
+        ir = jp + jm + 1
+        ip = np.floor(ttp * ir).astype(np.int64)
+        ip = np.mod(ip, 4 * ir)
+        out[pc] = np.where(z[pc] > 0, 2 * ir * (ir - 1) + ip, 12 * N * N - 2 * ir * (ir + 1) + ip)
+    return out

@@ -600,3 +600,73 @@ def test_multi_plane_source_order(cosmo):
     conv.add_plane(np.zeros(4), 1.0)
     with pytest.raises(ValueError, match="source redshift must be increasing"):
         conv.add_plane(np.zeros(4), 1.0)
+
+
+# ---- points -----------------------------------------------------------------
+
+
+@pytest.mark.parametrize("nside", [1, 2, 3, 4, 8, 16])
+def test_ring_geometry_of_points(nside):
+    """ang2pix_ring inverts the pixel centres of the ring geometry, and the
+    face coordinates of ring_to_xyf at (ix + 1/2, iy + 1/2) land on them."""
+    th, ph = O.pixel_angles(nside)
+    assert np.array_equal(O.ang2pix_ring(nside, th, ph), np.arange(12 * nside**2))
+    f, ix, iy = O.ring_to_xyf(nside, np.arange(12 * nside**2))
+    assert np.all((ix >= 0) & (ix < nside) & (iy >= 0) & (iy < nside))
+    lon, lat = O.face_lonlat(f, (ix + 0.5) / nside, (iy + 0.5) / nside)
+    assert np.allclose(np.radians(90 - lat), th, atol=1e-12)
+    d = np.mod(np.radians(lon) - ph + np.pi, 2 * np.pi) - np.pi
+    assert np.allclose(d, 0, atol=1e-12)
+
+
+def test_device_poisson_moments_and_limits():
+    lam = np.array([0.0, 0.5, 3.0, 31.9, 32.0, 33.0, 100.5])
+    c = O.device_poisson(np.repeat(lam, 4000), 11, 3).reshape(len(lam), 4000)
+    assert np.all(c[0] == 0) and c.dtype == np.int64
+    se = np.sqrt(lam[1:] / 4000)
+    assert np.all(np.abs(c[1:].mean(1) - lam[1:]) < 5 * se)
+    assert np.all(np.abs(c[1:].var(1) / lam[1:] - 1) < 0.15)
+    # deterministic in (seed, population, pixel)
+    assert np.array_equal(O.device_poisson(lam, 11, 3), O.device_poisson(lam, 11, 3))
+    assert not np.array_equal(O.device_poisson(np.full(64, 5.0), 11, 3),
+                              O.device_poisson(np.full(64, 5.0), 11, 4))
+
+
+def test_device_positions_in_their_pixels_and_uniform():
+    """Every point lies in its pixel; inside one nside-1 pixel the points fall
+    evenly on its 64 nside-8 sub-pixels (chi-square)."""
+    nside = 4
+    counts = np.random.default_rng(1).integers(0, 5, 12 * nside**2)
+    lon, lat, ipix = O.device_positions(nside, counts, 42, 0)
+    assert lon.size == counts.sum() and np.all((lon >= 0) & (lon < 360))
+    assert np.array_equal(O.ang2pix_ring(nside, np.radians(90 - lat), np.radians(lon)), ipix)
+    counts = np.zeros(12, np.int64)
+    counts[5] = 64000
+    lon, lat, _ = O.device_positions(1, counts, 42, 0)
+    sub = O.ang2pix_ring(8, np.radians(90 - lat), np.radians(lon))
+    h = np.bincount(sub, minlength=768)
+    occ = h[h > 0]
+    assert occ.size == 64
+    chi2 = np.sum((occ - 1000.0) ** 2 / 1000.0)
+    assert chi2 < 120  # 63 degrees of freedom
+
+
+def test_expected_counts_reference_cases():
+    """glass/points.py:214-348 on the reference test's cases
+    (tests/core/test_points.py:235-386): delta = -1 gives no points, a zero
+    contrast gives ngal per arcmin^2, vis scales, negative counts are clipped."""
+    npix = 12 * 16**2
+    n = O.expected_counts(np.full(npix, -1.0), 1e-3, None)
+    assert np.all(n == 0)
+    n = O.expected_counts(np.zeros(npix), 2e-3, 0.8, np.ones(npix))
+    assert np.allclose(n.sum(), O.ARCMIN2_SPHERE * 2e-3, rtol=1e-12)
+    vis = np.ones(npix)
+    vis[: npix // 2] = 0
+    assert np.all(O.expected_counts(np.zeros(npix), 1e-3, 0.8, vis)[: npix // 2] == 0)
+    n = O.expected_counts(np.full(npix, -3.0), 1e-3, 1.0)
+    assert np.all(n == 0)
+    d = np.random.default_rng(0).lognormal(0, 0.3, npix) - 1
+    n = O.expected_counts(d, 1e-3, 0.8, remove_monopole=True)
+    assert np.allclose(n.sum(), O.ARCMIN2_SPHERE * 1e-3, rtol=1e-10)
+    ll = O.expected_counts(d, 1e-3, 1.0, model="loglinear")
+    assert np.allclose(ll, O.expected_counts(d, 1e-3, None), rtol=1e-12)
