@@ -95,6 +95,10 @@ _SIGNATURES = {
                          _i64, _vp, _vp, _vp, _vp, _sz, _vp],
     "ga_points_scan": [_i64, _vp, _vp, _vp, _sz, _vp],
     "ga_points_positions": [_i32, _vp, _u64, _i64, _vp, _vp, _vp, _sz, _vp],
+    "ga_legendre_nodes": [_i32, _vp, _vp, _vp],
+    "ga_cl2corr": [_i32, _vp, _i32, _vp, _vp, _vp],
+    "ga_corr2cl_workspace_size": [_i32, _i32, ctypes.POINTER(_sz)],
+    "ga_corr2cl": [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
 }
 
 _lib = None

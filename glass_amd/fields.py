@@ -314,3 +314,83 @@ def _glass_to_healpix_alm(alm):
     _lib.call("ga_glass_to_healpix_alm_f64", nl, _lib.ptr(a), _lib.ptr(out), _lib.stream_ptr())
     return _lib.from_device(out, kind)
 
+
+
+# ---------------------------------------------------------------------------
+# Spectra preparation (glass/fields.py:239-300, :562-620, :745-836)
+
+
+def enumerate_spectra(entries):
+    """glass/fields.py:563-580: (i, j, entry) in standard order."""
+    for k, cl in enumerate(entries):
+        i = int((2 * k + 0.25) ** 0.5 - 0.5)
+        j = i * (i + 3) // 2 - k
+        yield i, j, cl
+
+
+def spectra_indices(n: int, *, xp=None):
+    """glass/fields.py:583-605: rows (i, j) in standard order."""
+    i, j = np.tril_indices(n)
+    return np.transpose([i, i - j])
+
+
+def compute_gaussian_spectra(fields, spectra):
+    """glass/fields.py:745-775: band-limited Gaussian spectra (grf.compute)."""
+    n = len(fields)
+    if len(spectra) != n * (n + 1) // 2:
+        raise ValueError("mismatch between number of fields and spectra")
+    gls = []
+    for i, j, cl in enumerate_spectra(spectra):
+        cl = np.asarray(cl)
+        gls.append(_grf.compute(cl, fields[i], fields[j]) if cl.shape[0] > 0 else 0 * cl)
+    return gls
+
+
+def solve_gaussian_spectra(fields, spectra):
+    """glass/fields.py:778-836: Gaussian spectra from the iterative solver
+    (grf.solve with zero padding 2 n, the monopole fixed to zero where the
+    target monopole is zero; a warning where it did not converge)."""
+    n = len(fields)
+    if len(spectra) != n * (n + 1) // 2:
+        raise ValueError("mismatch between number of fields and spectra")
+    if len(spectra) == 0:
+        return []
+    gls = []
+    for i, j, cl in enumerate_spectra(spectra):
+        cl = np.asarray(cl, dtype=np.float64)
+        if cl.shape[0] > 0:
+            monopole = 0.0 if cl[0] == 0 else None
+            gl, _cl_out, info = _grf.solve(cl, fields[i], fields[j], pad=2 * cl.shape[0],
+                                           monopole=monopole)
+            if info == 0:
+                warnings.warn(f"Gaussian spectrum for fields ({i}, {j}) did not converge",
+                              stacklevel=2)
+        else:
+            gl = 0 * cl
+        gls.append(gl)
+    return gls
+
+
+def discretized_cls(cls, *, lmax: int | None = None, ncorr: int | None = None,
+                    nside: int | None = None):
+    """glass/fields.py:239-300: truncate to lmax, keep ncorr correlations, and
+    multiply by the squared pixel window of nside (healpix.pixwin, computed on
+    the GPU from its definition)."""
+    if len(cls) == 0:
+        return []
+    if ncorr is not None:
+        n = nfields_from_nspectra(len(cls))
+        cls = [cls[i * (i + 1) // 2 + j] if j <= ncorr else np.asarray([])
+               for i in range(n) for j in range(i + 1)]
+    pw = _hp.pixwin(nside, lmax=lmax) if nside is not None else None
+    gls = []
+    for cl in cls:
+        cl = np.asarray(cl)
+        if cl.shape[0] > 0:
+            if lmax is not None:
+                cl = cl[: lmax + 1]
+            if pw is not None:
+                n = min(cl.shape[0], pw.shape[0])
+                cl = cl[:n] * pw[:n] ** 2
+        gls.append(cl)
+    return gls

@@ -303,6 +303,19 @@ int ga_points_positions(int32_t nside, const int64_t *counts, uint64_t seed, int
                         double *lon, double *lat, void *workspace, size_t ws_bytes,
                         void *stream);
 
+/* Angular power spectrum <-> correlation function on n Gauss-Legendre nodes
+ * (x ascending in (-1, 1), weights w; float64 device arrays), exact for
+ * l < n:  corr(x_k) = sum_l (2l+1)/(4 pi) cl_l P_l(x_k),
+ *         cl_l = 2 pi sum_k w_k corr(x_k) P_l(x_k).
+ * Stand in for transformcl.cltocorr / corrtocl as called by the Gaussian
+ * spectra solver (glass/grf/_solver.py:96-140, glass/grf/_core.py:184). */
+int ga_legendre_nodes(int32_t n, double *x, double *w, void *stream);
+int ga_cl2corr(int32_t nl, const double *cl, int32_t n, const double *x, double *corr,
+               void *stream);
+int ga_corr2cl_workspace_size(int32_t n, int32_t nl, size_t *bytes);
+int ga_corr2cl(int32_t n, const double *x, const double *w, const double *corr, int32_t nl,
+               double *cl, void *workspace, size_t ws_bytes, void *stream);
+
 /* Pointwise transforms on a float64 map (n pixels), out may equal x.
  * glass/grf/_transformations.py. */
 int ga_transform_f64(int64_t n, const double *x, const ga_epilogue *epi_host, double *out,

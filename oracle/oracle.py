@@ -865,3 +865,65 @@ def ang2pix_ring(nside: int, theta, phi) -> np.ndarray:
         ip = np.mod(ip, 4 * ir)
         out[pc] = np.where(z[pc] > 0, 2 * ir * (ir - 1) + ip, 12 * N * N - 2 * ir * (ir + 1) + ip)
     return out
+
+
+# --------------------------------------------------------------------------
+# C_l <-> C(theta) on Gauss-Legendre nodes (the transforms the device runs for
+# the Gaussian-spectra solver, glass/grf/_solver.py:27-148, in place of the
+# transformcl package): exact for l < n.
+
+
+def _legendre_table(x: np.ndarray, n: int) -> np.ndarray:
+    P = np.zeros((n, x.size))
+    P[0] = 1.0
+    if n > 1:
+        P[1] = x
+    for l in range(2, n):
+        P[l] = ((2 * l - 1) * x * P[l - 1] - (l - 1) * P[l - 2]) / l
+    return P
+
+
+def gl_nodes(n: int) -> tuple[np.ndarray, np.ndarray]:
+    """Gauss-Legendre nodes (ascending) and weights by Newton's method on P_n
+    in extended precision (numpy's ``longdouble``), rounded to float64.
+    numpy's ``leggauss`` evaluates the derivative at the unpolished root, so
+    its weights next to +-1 carry relative errors ~1e-10 at n ~ 300; these
+    carry the rounding of the last step only."""
+    ld = np.longdouble
+    i = np.arange((n + 1) // 2)
+    z = np.cos(np.pi * (i + ld(0.75)) / (n + ld(0.5))).astype(ld)
+
+    def pn(z):
+        p0, p1 = np.ones_like(z), z.copy()
+        for k in range(2, n + 1):
+            p0, p1 = p1, ((2 * k - 1) * z * p1 - (k - 1) * p0) / k
+        return p1, n * (z * p1 - p0) / (z * z - 1)
+
+    for _ in range(100):
+        p, dp = pn(z)
+        dz = p / dp
+        z = z - dz
+        if np.max(np.abs(dz)) < 1e-19:
+            break
+    _, dp = pn(z)
+    w = 2 / ((1 - z * z) * dp * dp)
+    x = np.concatenate([-z, z[::-1][n % 2:]]).astype(np.float64)
+    wt = np.concatenate([w, w[::-1][n % 2:]]).astype(np.float64)
+    if n % 2:
+        x[n // 2] = 0.0
+    return x, wt
+
+
+def gl_cltocorr(cl) -> np.ndarray:
+    cl = np.asarray(cl, dtype=np.float64)
+    n = cl.size
+    x, _ = gl_nodes(n)
+    ell = np.arange(n)
+    return ((2 * ell + 1) / (4 * np.pi) * cl) @ _legendre_table(x, n)
+
+
+def gl_corrtocl(corr) -> np.ndarray:
+    corr = np.asarray(corr, dtype=np.float64)
+    n = corr.size
+    x, w = gl_nodes(n)
+    return 2 * np.pi * (_legendre_table(x, n) @ (w * corr))
