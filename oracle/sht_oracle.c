@@ -463,6 +463,161 @@ void oracle_alm2map_rings(int64_t nside, int64_t lmax, const double *alm, double
   free(F); free(seeds); free(tab); free(rings);
 }
 
+/* The same synthesis with the equatorial symmetry a tuned CPU SHT uses (the
+ * CPU-baseline form): one recurrence per (m, north ring) serves the ring and
+ * its south mirror, lambda_lm(-x) = (-1)^(l+m) lambda_lm(x), so the l-sums are
+ * split by the parity of l - m into E and O and F_north = E + O,
+ * F_south = E - O.  A ring's value is carried as p 2^e while it is below
+ * 2^-900 (where it cannot change an FP64 sum of order one and is left out);
+ * the scale is folded in as soon as it is above, and from the l where every
+ * ring of the block is folded the recurrence runs unscaled, two l at a time. */
+#define SYM_BLK 16
+static void synth_block_sym(int64_t m, int64_t lmax, const double *x, const scaled_t *sd,
+                            int nb, const double *ab, const double *a, double *fn, double *fs) {
+  double p0[SYM_BLK], p1[SYM_BLK], er[SYM_BLK], ei[SYM_BLK], orr[SYM_BLK], oi[SYM_BLK];
+  int64_t e[SYM_BLK];
+  for (int j = 0; j < SYM_BLK; ++j) {
+    const int jj = j < nb ? j : 0;
+    p0[j] = 0.0; p1[j] = sd[jj].v; e[j] = sd[jj].e;
+    er[j] = ei[j] = orr[j] = oi[j] = 0.0;
+  }
+  const double big = 0x1p300, rescale = 0x1p-600;
+  int64_t l = m;
+  int any_low = 1;
+  for (; l <= lmax && any_low; ++l) {
+    const double c0 = ab[2 * (l - m)], c1 = ab[2 * (l - m) + 1];
+    const int odd = (int)((l - m) & 1);
+    any_low = 0;
+    for (int j = 0; j < SYM_BLK; ++j) {
+      if (l > m) {
+        double pn = c0 * (x[j] * p1[j] - c1 * p0[j]);
+        p0[j] = p1[j]; p1[j] = pn;
+      }
+      if (e[j] != 0) {
+        if (fabs(p1[j]) > big) { p1[j] *= rescale; p0[j] *= rescale; e[j] += 600; }
+        /* fold the scale in once the value is well inside the normal range */
+        if (p1[j] != 0.0 && ilogb(p1[j]) + e[j] > -900) {
+          p1[j] = ldexp(p1[j], (int)e[j]);
+          p0[j] = ldexp(p0[j], (int)e[j]);
+          e[j] = 0;
+        } else {
+          any_low = 1; /* below 2^-900: contributes nothing at FP64 */
+          continue;
+        }
+      }
+      const double v = p1[j];
+      if (odd) { orr[j] += v * a[2 * (l - m)]; oi[j] += v * a[2 * (l - m) + 1]; }
+      else { er[j] += v * a[2 * (l - m)]; ei[j] += v * a[2 * (l - m) + 1]; }
+    }
+  }
+  /* plain phase, two steps (one even, one odd l - m) at a time */
+  if (l <= lmax && ((l - m) & 1)) {
+    const double c0 = ab[2 * (l - m)], c1 = ab[2 * (l - m) + 1];
+    for (int j = 0; j < SYM_BLK; ++j) {
+      double pn = c0 * (x[j] * p1[j] - c1 * p0[j]);
+      p0[j] = p1[j]; p1[j] = pn;
+      orr[j] += pn * a[2 * (l - m)]; oi[j] += pn * a[2 * (l - m) + 1];
+    }
+    ++l;
+  }
+  for (; l + 1 <= lmax; l += 2) {
+    const double c0 = ab[2 * (l - m)], c1 = ab[2 * (l - m) + 1];
+    const double d0 = ab[2 * (l + 1 - m)], d1 = ab[2 * (l + 1 - m) + 1];
+    const double br = a[2 * (l - m)], bi = a[2 * (l - m) + 1];
+    const double cr = a[2 * (l + 1 - m)], ci = a[2 * (l + 1 - m) + 1];
+    for (int j = 0; j < SYM_BLK; ++j) {
+      const double pe = c0 * (x[j] * p1[j] - c1 * p0[j]);
+      const double po = d0 * (x[j] * pe - d1 * p1[j]);
+      p0[j] = pe; p1[j] = po;
+      er[j] += pe * br; ei[j] += pe * bi;
+      orr[j] += po * cr; oi[j] += po * ci;
+    }
+  }
+  if (l <= lmax) {
+    const double c0 = ab[2 * (l - m)], c1 = ab[2 * (l - m) + 1];
+    for (int j = 0; j < SYM_BLK; ++j) {
+      const double pe = c0 * (x[j] * p1[j] - c1 * p0[j]);
+      er[j] += pe * a[2 * (l - m)]; ei[j] += pe * a[2 * (l - m) + 1];
+    }
+  }
+  for (int j = 0; j < nb; ++j) {
+    fn[2 * j] = er[j] + orr[j]; fn[2 * j + 1] = ei[j] + oi[j];
+    fs[2 * j] = er[j] - orr[j]; fs[2 * j + 1] = ei[j] - oi[j];
+  }
+}
+
+void oracle_alm2map_sym(int64_t nside, int64_t lmax, const double *alm, double *map,
+                        int nthreads) {
+  const int64_t nr = nrings_of(nside), nnorth = 2 * nside; /* rings 1 .. 2 nside */
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  int64_t *rings = malloc((size_t)nnorth * sizeof(int64_t));
+  double *x = malloc((size_t)nnorth * sizeof(double));
+  for (int64_t i = 0; i < nnorth; ++i) {
+    ring_t g;
+    rings[i] = i + 1;
+    ring_info(nside, i + 1, &g);
+    x[i] = g.z;
+  }
+  double *tab = legendre_table(lmax);
+  scaled_t *seeds = seed_table(nside, lmax, rings, nnorth);
+  /* F[ring - 1][m], all 4 nside - 1 rings */
+  double *F = malloc((size_t)nr * (size_t)(lmax + 1) * 2 * sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t m = 0; m <= lmax; ++m) {
+    const double *ab = tab + 2 * alm_index(lmax, m, m);
+    const double *a = alm + 2 * alm_index(lmax, m, m);
+    for (int64_t i0 = 0; i0 < nnorth; i0 += SYM_BLK) {
+      const int nb = (int)((nnorth - i0) < SYM_BLK ? (nnorth - i0) : SYM_BLK);
+      double xb[SYM_BLK], fn[2 * SYM_BLK], fs[2 * SYM_BLK];
+      scaled_t sd[SYM_BLK];
+      for (int j = 0; j < SYM_BLK; ++j) {
+        const int64_t i = i0 + (j < nb ? j : 0);
+        xb[j] = x[i];
+        sd[j] = seeds[(size_t)i * (size_t)(lmax + 1) + (size_t)m];
+      }
+      synth_block_sym(m, lmax, xb, sd, nb, ab, a, fn, fs);
+      for (int j = 0; j < nb; ++j) {
+        const int64_t rn = i0 + j + 1, rs = 4 * nside - rn;
+        double *f = F + 2 * ((size_t)(rn - 1) * (size_t)(lmax + 1) + (size_t)m);
+        f[0] = fn[2 * j];
+        f[1] = (m == 0) ? 0.0 : fn[2 * j + 1];
+        if (rs != rn) {
+          f = F + 2 * ((size_t)(rs - 1) * (size_t)(lmax + 1) + (size_t)m);
+          f[0] = fs[2 * j];
+          f[1] = (m == 0) ? 0.0 : fs[2 * j + 1];
+        }
+      }
+    }
+  }
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *Fr = malloc((size_t)(lmax + 1) * sizeof(double));
+    double *Fi = malloc((size_t)(lmax + 1) * sizeof(double));
+    double *wr = malloc((size_t)(16 * nside) * sizeof(double));
+    double *wi = malloc((size_t)(16 * nside) * sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+    for (int64_t r = 1; r <= nr; ++r) {
+      ring_t g;
+      ring_info(nside, r, &g);
+      for (int64_t m = 0; m <= lmax; ++m) {
+        Fr[m] = F[2 * ((size_t)(r - 1) * (size_t)(lmax + 1) + (size_t)m)];
+        Fi[m] = F[2 * ((size_t)(r - 1) * (size_t)(lmax + 1) + (size_t)m) + 1];
+      }
+      ring_synth(&g, lmax, Fr, Fi, map + g.start, wr, wi);
+    }
+    free(Fr); free(Fi); free(wr); free(wi);
+  }
+  free(F); free(seeds); free(tab); free(x); free(rings);
+}
+
 void oracle_alm2map(int64_t nside, int64_t lmax, const double *alm, double *map,
                     int nthreads) {
   oracle_alm2map_rings(nside, lmax, alm, map, 1, 1, nthreads);
