@@ -73,6 +73,12 @@ _SIGNATURES = {
     "ga_legendre_contract_batch_f32": [_vp, _i32, _vp, _vp, _sz, _vp],
     "ga_legendre_prepare_colmax_batch_f32": [_vp, _i32, _vp, _vp, _i64, _vp, _sz, _vp],
     "ga_ring_synth_batch_f32": [_vp, _i32, _vp, _vp, _vp, _vp],
+    "ga_legendre_contract_grouped_f32": [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_i32),
+                                         ctypes.POINTER(_vp), ctypes.POINTER(_sz), _vp],
+    "ga_graph_begin": [_vp],
+    "ga_graph_end": [_vp, ctypes.POINTER(_vp)],
+    "ga_graph_launch": [_vp, _vp],
+    "ga_graph_destroy": [_vp],
     "ga_map2alm_f64": [_vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
     "ga_map2alm_batch_f32": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
     "ga_alm2map_spin_f64": [_vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp],
@@ -220,6 +226,47 @@ class Plan:
                 _lib.ga_plan_destroy(self.h)
         except Exception:  # noqa: BLE001
             pass
+
+
+class Graph:
+    """Work recorded from one stream through the library's graph entry points
+    (``ga_graph_begin`` / ``ga_graph_end``) and replayed with one call."""
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+
+    def capture(self, fn):
+        """Record what ``fn()`` enqueues on the current torch stream (which
+        must not be the default stream) and on the streams it joins."""
+        s = stream_ptr()
+        call("ga_graph_begin", s)
+        try:
+            fn()
+        finally:
+            rc = lib().ga_graph_end(s, ctypes.byref(self.h))
+        check(rc, "ga_graph_end")
+        return self
+
+    def launch(self):
+        call("ga_graph_launch", self.h, stream_ptr())
+
+    def __del__(self):
+        try:
+            if self.h and _lib is not None:
+                _lib.ga_graph_destroy(self.h)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def contract_grouped(plans, nshells, workspaces) -> None:
+    """``ga_legendre_contract_grouped_f32`` over up to 8 (plan, S, prepared
+    workspace) triples, on the current stream."""
+    n = len(plans)
+    hp = (_vp * n)(*[p.h.value for p in plans])
+    ns = (_i32 * n)(*[int(x) for x in nshells])
+    wp = (_vp * n)(*[w.data_ptr() for w in workspaces])
+    wb = (_sz * n)(*[int(w.numel()) for w in workspaces])
+    call("ga_legendre_contract_grouped_f32", n, hp, ns, wp, wb, stream_ptr())
 
 
 _plans: dict[tuple[int, int, int], Plan] = {}

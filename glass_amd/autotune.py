@@ -1,8 +1,8 @@
 """Autotuner of the batched MFMA Legendre stage.
 
-For one geometry and batch width it times the candidate kernel
-configurations -- the split-FP16 path with one or two 32-column tiles per
-wave, and the FP32 MFMA path -- on the engine's own alm (HIP events on the
+For one geometry and batch width it times the candidate tile shapes of the
+Legendre contraction -- split FP16 with one or two 32-column tiles per wave,
+and the FP32 MFMA path with two tiles -- on the engine's own alm (HIP events on the
 launch stream, median of ``reps``), keeps the fastest, and caches the choice
 per (nside, lmax, shells) in a JSON file (``GLASS_AMD_TUNE_CACHE``, default
 ``~/.cache/glass_amd/legendre_tune.json``).  The choice is stored in the
@@ -21,8 +21,11 @@ import numpy as np
 
 from . import _lib
 
-# (path, tiles): path 2 = split FP16, 1 = FP32 MFMA
+# (path, tiles): path 2 = split FP16 with one or two 32-column tiles per wave,
+# path 1 = FP32 MFMA with two tiles
 CANDIDATES = ((2, 1), (2, 2), (1, 2))
+# the tile shapes a grouped (several geometries in one grid) launch runs
+GROUPED_CANDIDATES = ((2, 1), (2, 2))
 
 
 def cache_path() -> pathlib.Path:
@@ -71,6 +74,8 @@ def tune_legendre(eng, candidates=CANDIDATES, reps: int = 3, use_cache: bool = T
     stage_legendre) and store it in the engine's plan."""
     T = _lib.torch()
     key = f"{eng.nside}:{eng.lmax}:{eng.S}"
+    if tuple(candidates) != CANDIDATES:
+        key += ":" + ",".join(f"{p}/{t}" for p, t in candidates)
     cache = _load() if use_cache else {}
     if key in cache:
         apply(eng.plan, cache[key])

@@ -175,6 +175,23 @@ extern "C" int ga_legendre_contract_batch_f32(const ga_plan *p, int32_t S, const
   return launch_legendre_contract_batch(p, p->tune, S, alm, ws, (hipStream_t)stream);
 }
 
+extern "C" int ga_legendre_contract_grouped_f32(int32_t ngroup, const ga_plan *const *plans,
+                                                const int32_t *nshell, void *const *workspace,
+                                                const size_t *ws_bytes, void *stream) {
+  clear_error();
+  GA_REQUIRE(ngroup >= 1 && ngroup <= kGroupedMax && plans && nshell && workspace && ws_bytes,
+             GA_ERR_INVALID, "invalid argument");
+  for (int i = 0; i < ngroup; ++i) {
+    GA_PLAN_OK(plans[i]);
+    GA_REQUIRE(nshell[i] >= 1 && workspace[i], GA_ERR_INVALID, "invalid argument (geometry %d)",
+               i);
+    const size_t need = f32_workspace(plans[i], nshell[i]).total;
+    GA_REQUIRE(ws_bytes[i] >= need, GA_ERR_WORKSPACE, "workspace %d: %zu < %zu bytes", i,
+               ws_bytes[i], need);
+  }
+  return launch_legendre_contract_grouped(ngroup, plans, nshell, workspace, (hipStream_t)stream);
+}
+
 extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float *F, float *maps,
                                        const ga_epilogue *epi, void *stream) {
   GA_PLAN_OK(p);
@@ -285,4 +302,53 @@ extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t
   int rc = launch_spin_synth_f16(p, spin, S, E, B, ws, s);
   if (rc) return rc;
   return launch_ring_synth_f32(p, 2 * S, kMfmaGroup, (const float *)ws, maps, nullptr, s);
+}
+
+// ---- graph capture ------------------------------------------------------------
+
+struct ga_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" int ga_graph_begin(void *stream) {
+  clear_error();
+  GA_REQUIRE(stream, GA_ERR_INVALID, "capture needs a created stream, not the null stream");
+  GA_CHECK_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return GA_OK;
+}
+
+extern "C" int ga_graph_end(void *stream, ga_graph **out) {
+  clear_error();
+  GA_REQUIRE(stream && out, GA_ERR_INVALID, "invalid argument");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  GA_CHECK_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  GA_REQUIRE(g, GA_ERR_HIP, "the capture recorded no graph");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return GA_ERR_HIP;
+  }
+  *out = new ga_graph{g, exec};
+  return GA_OK;
+}
+
+extern "C" int ga_graph_launch(const ga_graph *graph, void *stream) {
+  clear_error();
+  GA_REQUIRE(graph, GA_ERR_INVALID, "invalid argument");
+  GA_CHECK_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  return GA_OK;
+}
+
+extern "C" int ga_graph_destroy(ga_graph *graph) {
+  clear_error();
+  if (!graph) return GA_OK;
+  const hipError_t e1 = hipGraphExecDestroy(graph->exec);
+  const hipError_t e2 = hipGraphDestroy(graph->graph);
+  delete graph;
+  GA_REQUIRE(e1 == hipSuccess && e2 == hipSuccess, GA_ERR_HIP, "graph destroy failed");
+  return GA_OK;
 }

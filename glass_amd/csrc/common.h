@@ -54,9 +54,10 @@ struct ga_tuning {
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
   int f16_flush = 0;    // split-FP16 synthesis accumulation: n > 0 master sums in registers
-                        // flushed every n slabs (one column tile), -k (k >= 2) k l-chunks
-                        // summed in the ring buffer, -1 one chain, 0 auto (kAutoFlushSlabs
-                        // when lmax > kAutoFlushLmax, else kAutoChunks)
+                        // flushed every n slabs (one column tile), -2 two l-chunks (the
+                        // first parked in the ring buffer), -1 one chain, 0 auto
+                        // (kAutoFlushSlabs when lmax > kAutoFlushLmax or the parking does
+                        // not fit the batch, else kAutoChunks)
 };
 // lmax above which the split-FP16 synthesis flushes its MFMA accumulators into
 // register master sums by default, the flush period in slabs, and the default
@@ -81,7 +82,10 @@ struct f32_ws_layout {
 f32_ws_layout f32_workspace(const ga_plan *p, int nbatch);
 // the accumulation form the split-FP16 synthesis of plan p runs under tuning
 // set tu: the f16_flush knob with 0 (automatic) resolved
-int f16_accumulation(const ga_plan *p, const ga_tuning &tu);
+int f16_accumulation(const ga_plan *p, const ga_tuning &tu, int nbatch);
+// whether the two-chunk form can park in the ring buffer of nbatch shells
+// (64 rows of it must span less than 4 GiB: 32-bit buffer offsets)
+bool f16_park_fits(const ga_plan *p, int nbatch);
 
 // ---- sizes ----------------------------------------------------------------
 inline int64_t nalm_of(int64_t lmax) { return (lmax + 1) * (lmax + 2) / 2; }

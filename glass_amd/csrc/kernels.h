@@ -25,10 +25,11 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
                                 const double *csc = nullptr);
 // nt: column tiles per wave (0 automatic, 1, 2, 4 = wave pairs); flush: the
 // accumulation form (f16_accumulation: -1 one chain, n > 0 register master
-// sums every n slabs, -k k l-chunks with per-wave partial sums in partials)
+// sums every n slabs, -2 two l-chunks, the first parked in F, the O half of
+// its equator row in eqo: (lmax + 1) 2 S floats)
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s, int nt, int flush,
-                                 void *partials);
+                                 float *eqo);
 // the batched Legendre stage on the path chosen by the tuning set tu (a
 // snapshot of the plan's, the same for the preparation and the contraction
 // of one synthesis); ws is the FP32 workspace with F at its start
@@ -40,6 +41,12 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
                                    const float *alm, void *ws, hipStream_t s);
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
                                 hipStream_t s);
+// the split-FP16 contraction of up to kGroupedMax geometries (plans[i], S[i]
+// shells, prepared FP32 workspace ws[i]) in one grid, each on its plan's
+// tiles and accumulation form
+constexpr int kGroupedMax = 8;
+int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, const int32_t *S,
+                                     void *const *ws, hipStream_t s);
 
 // Ring stage of synthesis: F -> maps [S][npix] with fused transform.  F holds
 // the shells in groups of G: offset ((ring ngrp + s/G)(lmax+1) + m) G + s%G

@@ -308,6 +308,88 @@ __device__ __forceinline__ int skip_dead_slabs(double (*cb)[kSlab],
   return s;
 }
 
+// Two-chunk parking in the ring buffer.  The first chunk's raw accumulators
+// of a wave are stored where the wave's own output goes: E of (row, col) in
+// F at the north ring, O at the south ring (as store_tile addresses them;
+// every (ring, m, column) of F belongs to exactly one wave), O of the
+// equator row, whose north and south ring coincide, in eqo[m ncols + col].
+// At the end each lane loads back what it stored (program order, same
+// addresses) and adds it before the epilogue, which then overwrites F.  The
+// result is the same as with the partial sums kept elsewhere; F itself is
+// the parking space, so the chunked form needs no workspace beyond
+// (lmax + 1) ncols floats.
+template <int NT, bool LOAD>
+__device__ __forceinline__ void park_tile(f32x16 (&acc)[2][2][NT], int m, int lmax, int npairs,
+                                          int nrings, int ncols, int colbase, int rp0, int lane,
+                                          float *__restrict__ F, float *__restrict__ eqo) {
+  const int half = lane >> 5, col_in = lane & 31;
+  // Buffer addressing from two wave-uniform bases, so that a store costs no
+  // address registers: north row rp0 + r at byte r rstride, south row
+  // nrings - 1 - rp0 - r at byte (63 - r) rstride of its base; the column
+  // part is per lane, the row part per element (scalar).  The host parks in
+  // F only where 64 rows span less than 4 GiB.
+  const uint32_t rstride = uint32_t((ncols + 7) / 8) * uint32_t(lmax + 1) * 32u;
+  const int r0 = __builtin_amdgcn_readfirstlane(rp0);
+  char *const fnb = reinterpret_cast<char *>(F) + int64_t(r0) * rstride + int64_t(m) * 32;
+  char *const fsb =
+      reinterpret_cast<char *>(F) + (int64_t(nrings - 1 - r0) - 63) * rstride + int64_t(m) * 32;
+  const __amdgpu_buffer_rsrc_t rn = __builtin_amdgcn_make_buffer_rsrc(fnb, 0, -1, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(fsb, 0, -1, 0x00020000);
+#pragma unroll
+  for (int ct = 0; ct < NT; ++ct) {
+    const int col = colbase + ct * 32 + col_in;
+    const bool colok = col < ncols;
+    const uint32_t cofs = uint32_t(col >> 3) * uint32_t(lmax + 1) * 32u + uint32_t(col & 7) * 4u;
+    const uint32_t vn = cofs + uint32_t(4 * half) * rstride;
+    const uint32_t vs = cofs + uint32_t(4 * (1 - half)) * rstride;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int rr = 32 * t + 8 * (e >> 2) + (e & 3);  // row without the 4 half part
+        const int rpo = r0 + rr + 4 * half;
+        const int sn = rr * int(rstride), ss = (59 - rr) * int(rstride);
+        if (colok && rpo < npairs) {
+          if (LOAD) {
+            acc[0][t][ct][e] += __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rn, vn, sn, 0));
+            acc[1][t][ct][e] +=
+                rpo < npairs - 1
+                    ? __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vs, ss, 0))
+                    : eqo[int64_t(m) * ncols + col];
+          } else {
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc[0][t][ct][e]), rn, vn, sn, 0);
+            if (rpo < npairs - 1)
+              __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(acc[1][t][ct][e]), rs, vs, ss,
+                                                    0);
+            else
+              eqo[int64_t(m) * ncols + col] = acc[1][t][ct][e];
+          }
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// One geometry of the split-FP16 contraction: plan tables, the prepared B
+// operands and column exponents, the ring buffer F and the parking space of
+// the equator row's O half (two-chunk form).
+struct f16_geom {
+  const double *z, *sth, *gam, *cm;
+  const int32_t *ring_pnum, *ring_pden;
+  const int64_t *slab_off;
+  const uint4 *B;
+  const int32_t *colexp;
+  float *F;
+  float *eqo;
+  int lmax, npairs, nrings, ncols, nct, nrg, ncg, lam_exp, flush, nchunk, nt, blk_begin;
+};
+
+struct f16_group {
+  f16_geom g[kGroupedMax];
+  int ngroup;
+};
+
 // Epilogue of a wave's accumulators (as in the FP32 stage): D layout of
 // 32x32: lane L, element e -> row 8 (e/4) + 4 (L/32) + e%4, column L % 32.
 // Unscale by 2^-(8 + shift of the column), north = E + O, south = E - O,
@@ -375,18 +457,29 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
 // sqrt(1 / (3 flush) + flush / nslab), which the long sums of lmax > 4096
 // need to stay within 1e-5 of the map RMS.  It doubles the accumulator
 // registers, so it is instantiated with one column tile per wave.
-template <int NT, int MINW, bool FLUSH>
-__global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
-    int lmax, int npairs, int nrings, int ncols, int nct, int nrg, int ncg,
-    const double *__restrict__ z, const double *__restrict__ sth, const double *__restrict__ gam,
-    const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
-    const int32_t *__restrict__ ring_pden, const int64_t *__restrict__ slab_off,
-    const uint4 *__restrict__ B, const int32_t *__restrict__ colexp, float *__restrict__ F,
-    int lam_exp, int flush, int nchunk, float4 *__restrict__ part) {
-  // XCD-aware decode: block b runs on XCD b % 8; XCD x takes m = x, x + 8, ...
-  // (largest l-range first), all (ring group, column group) tiles of one m
-  // back to back so they share that m's B slabs in the XCD's L2.
-  const int b = blockIdx.x;
+//
+// The body works on one geometry, described by an f16_geom, for block b of
+// that geometry's grid; the plain kernel runs one geometry, the grouped
+// kernel below several in one grid.
+template <int NT, bool FLUSH>
+__device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) {
+  const int lmax = g.lmax, npairs = g.npairs, nrings = g.nrings, ncols = g.ncols, nct = g.nct,
+            nrg = g.nrg, ncg = g.ncg, lam_exp = g.lam_exp, flush = g.flush, nchunk = g.nchunk;
+  const double *__restrict__ z = g.z;
+  const double *__restrict__ sth = g.sth;
+  const double *__restrict__ gam = g.gam;
+  const double *__restrict__ cm = g.cm;
+  const int32_t *__restrict__ ring_pnum = g.ring_pnum;
+  const int32_t *__restrict__ ring_pden = g.ring_pden;
+  const int64_t *__restrict__ slab_off = g.slab_off;
+  const uint4 *__restrict__ B = g.B;
+  const int32_t *__restrict__ colexp = g.colexp;
+  float *__restrict__ F = g.F;
+  float *__restrict__ eqo = g.eqo;
+  // XCD-aware decode: block b runs on XCD b % 8 (a geometry's blocks start at
+  // a multiple of 8 of the grid); XCD x takes m = x, x + 8, ... (largest
+  // l-range first), all (ring group, column group) tiles of one m back to
+  // back so they share that m's B slabs in the XCD's L2.
   const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
   const int m = 8 * (kq / G) + xcd;
   if (m > lmax) return;
@@ -428,14 +521,6 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
         if (FLUSH) mst[p][t][FLUSH ? ct : 0] = f32x16{};
       }
   int since_flush = 0;  // slabs accumulated since the last flush
-  // chunked form (nchunk >= 2): the l-range of the wave is cut into nchunk
-  // chunks of slabs; at each cut the accumulators are stored to the wave's
-  // slot of that chunk in memory (16 NT float4 per lane, coalesced) and
-  // restart from zero; the stored chunks are added back, in order, before
-  // the tile is written.  Stores only at the cuts: no loads there.
-  int chunk = 0, s_first = 0, span = 0;
-  float4 *__restrict__ mypart =
-      part + (int64_t(blockIdx.x) * 4 + wave) * int64_t(nchunk - 1) * (16 * NT * 64) + lane;
 
   // Recurrence coefficients go through LDS, one slab (32 double2) per wave
   // and buffer: the next slab's are loaded from global memory (lanes 0..31,
@@ -456,15 +541,9 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
   // matches the per-group checks bit for bit).  The slab in which some lane
   // reaches FP16 range is undone and run again by the full loop.
   int s = skip_dead_slabs(cbuf[wave], co, lastc, lane, x, m, lmax, nsl, p0, p1, k, cpre);
-  int cut_end = nsl;  // end of the current chunk of slabs
-  if (!FLUSH && nchunk >= 2 && nsl - s >= 4 * nchunk) {
-    s_first = s;
-    span = nsl - s;
-    cut_end = s_first + span / nchunk;
-  }
-
-  for (;;) {
-    for (; s < cut_end; ++s) {
+  // the slab loop from s to end (accumulating into acc)
+  auto run_slabs = [&](const int end) {
+    for (; s < end; ++s) {
       const int l0 = m + kSlab * s;
       const int cur = s & 1;
       // B operands of this slab, consumed after the recurrence below
@@ -504,27 +583,26 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
         }
       }
     }
-    if (FLUSH || s >= nsl) break;
-    // end of a chunk: store it to its slot and restart the accumulators
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int ct = 0; ct < NT; ++ct)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            mypart[((chunk * 16 * NT) + ((p * 2 + t) * NT + ct) * 4 + q) * 64] =
-                make_float4(acc[p][t][ct][4 * q], acc[p][t][ct][4 * q + 1],
-                            acc[p][t][ct][4 * q + 2], acc[p][t][ct][4 * q + 3]);
+  };
+
+  // Two-chunk form (nchunk == 2): the l-range left after the dead slabs is
+  // cut in two halves; the first half's accumulators are parked in F
+  // (park_tile), the second half starts from zero, and the parked sums are
+  // added back before the epilogue: two shorter FP32 rounding chains.  The
+  // two halves are two copies of the slab loop, not one loop with a cut in
+  // it, so the parking code sits outside every loop.
+  const bool two = !FLUSH && nchunk == 2 && nsl - s >= 8;
+  run_slabs(two ? s + (nsl - s) / 2 : nsl);
+  if (two) {
+    park_tile<NT, false>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
-    ++chunk;
-    cut_end = chunk + 1 < nchunk ? s_first + (chunk + 1) * span / nchunk : nsl;
+    run_slabs(nsl);
+    park_tile<NT, true>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
   }
   if (FLUSH) {
 #pragma unroll
@@ -534,35 +612,36 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] += mst[p][t][FLUSH ? ct : 0];
   }
-
-  if (!FLUSH && chunk > 0) {
-    // the stored chunks added to the last one (in registers), first to last,
-    // 16 values at a time (the scheduling barrier keeps the compiler from
-    // issuing every load at once, which would spill the accumulators)
-    for (int c = 0; c < chunk; ++c) {
-#pragma unroll
-      for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-          for (int ct = 0; ct < NT; ++ct) {
-            float4 o[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              o[q] = mypart[((c * 16 * NT) + ((p * 2 + t) * NT + ct) * 4 + q) * 64];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              acc[p][t][ct][4 * q] += o[q].x;
-              acc[p][t][ct][4 * q + 1] += o[q].y;
-              acc[p][t][ct][4 * q + 2] += o[q].z;
-              acc[p][t][ct][4 * q + 3] += o[q].w;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-          }
-    }
-  }
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
                  colexp, F, lam_exp);
+}
+
+template <int NT, int MINW, bool FLUSH>
+__global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(const f16_geom g) {
+  contract_blocks<NT, FLUSH>(g, int(blockIdx.x));
+}
+
+// Several geometries in one grid (an uneven batch): geometry i owns blocks
+// [blk_begin_i, blk_begin_{i+1}), largest first, so its long blocks are
+// dispatched first and the short blocks of the small geometries fill the
+// CUs as the large ones drain.  The geometry of a block is uniform, and so is
+// its tile count (one or two 32-column tiles per wave, both bodies are
+// compiled in).  One accumulation chain or l-chunks, no register master sums.
+template <int MINW>
+__global__ __launch_bounds__(256, MINW) void legendre_synth_f16_grouped_kernel(
+    const f16_group grp) {
+  const int bx = int(blockIdx.x);
+  int gi = 0;
+#pragma unroll
+  for (int i = 1; i < kGroupedMax; ++i)
+    if (i < grp.ngroup && bx >= grp.g[i].blk_begin) gi = i;
+  f16_geom g = grp.g[0];
+#pragma unroll
+  for (int i = 1; i < kGroupedMax; ++i)
+    if (i == gi) g = grp.g[i];
+  const int b = bx - g.blk_begin;
+  if (g.nt == 2) contract_blocks<2, false>(g, b);
+  else contract_blocks<1, false>(g, b);
 }
 
 // Producer / consumer form.  In the single-wave form every wave runs the
@@ -736,16 +815,52 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
   return GA_OK;
 }
 
+// the f16_geom of plan p for S shells (flush resolved, nt 1 or 2) and its
+// block count
+static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
+                             const int32_t *colexp, int nt, int flush, float *eqo,
+                             int64_t *nblk) {
+  f16_geom g{};
+  g.z = p->d_z;
+  g.sth = p->d_sth;
+  g.gam = p->d_gam;
+  g.cm = p->d_cm;
+  g.ring_pnum = p->d_ring_pnum;
+  g.ring_pden = p->d_ring_pden;
+  g.slab_off = p->d_slab_off;
+  g.B = reinterpret_cast<const uint4 *>(bws);
+  g.colexp = colexp;
+  g.F = F;
+  g.eqo = eqo;
+  g.lmax = p->lmax;
+  g.npairs = int(p->npairs);
+  g.nrings = int(p->nrings);
+  g.ncols = 2 * S;
+  g.nct = (g.ncols + 31) / 32;
+  g.nrg = int((p->npairs + 255) / 256);
+  g.ncg = (g.ncols + 32 * nt - 1) / (32 * nt);
+  g.lam_exp = p->lam_exp16;
+  g.flush = flush;
+  g.nchunk = flush == -2 ? 2 : 1;
+  g.nt = nt;
+  g.blk_begin = 0;
+  *nblk = int64_t(8) * ((p->lmax + 1 + 7) / 8) * g.nrg * g.ncg;
+  return g;
+}
+
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t stream, int nt, int flush,
-                                 void *partials) {
+                                 float *eqo) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
   // flush (resolved): n > 0 master sums in registers (they take the registers
-  // of a second tile); n <= -2: -n l-chunks with partial sums in memory
-  const int nchunk = flush <= -2 ? -flush : 1;
-  GA_REQUIRE(nchunk == 1 || partials, GA_ERR_INVALID, "chunked accumulation needs partials");
+  // of a second tile); -2: two l-chunks, the first parked in F
+  GA_REQUIRE(flush == -1 || flush == -2 || flush > 0, GA_ERR_INVALID,
+             "accumulation form %d out of range", flush);
+  GA_REQUIRE(flush != -2 || (eqo && f16_park_fits(p, S)), GA_ERR_UNSUPPORTED,
+             "two-chunk accumulation needs 64 ring rows of %d shells within 4 GiB; "
+             "use one chain (-1) or register master sums (n > 0)", S);
   if (flush > 0) nt = 1;
   if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : 2;
   if (nt == 4) {
@@ -762,21 +877,53 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
     GA_LAUNCH_CHECK();
     return GA_OK;
   }
-  const int nrg = (p->npairs + 255) / 256;
-  const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
-  const int nm8 = (p->lmax + 1 + 7) / 8;
-  const int64_t nblk2 = int64_t(8) * nm8 * nrg * ncg;
+  int64_t nblk2 = 0;
+  const f16_geom g = f16_geometry(p, S, F, bws, colexp, nt, flush, eqo, &nblk2);
   GA_REQUIRE(nblk2 < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
 #define GA_F16_LAUNCH(NT_, MW_, FL_)                                                            \
   hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_, FL_>), dim3(unsigned(nblk2)),        \
-                     dim3(256), 0, stream, p->lmax, p->npairs, p->nrings, ncols, nct, nrg, ncg, \
-                     p->d_z, p->d_sth, p->d_gam, p->d_cm, p->d_ring_pnum, p->d_ring_pden,       \
-                     p->d_slab_off, B, colexp, F, p->lam_exp16, flush, nchunk,               \
-                     static_cast<float4 *>(partials))
+                     dim3(256), 0, stream, g)
   if (flush > 0) GA_F16_LAUNCH(1, 2, true);
   else if (nt == 1) GA_F16_LAUNCH(1, 2, false);
   else GA_F16_LAUNCH(2, 2, false);
 #undef GA_F16_LAUNCH
+  GA_LAUNCH_CHECK();
+  return GA_OK;
+}
+
+int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, const int32_t *S,
+                                     void *const *ws, hipStream_t stream) {
+  GA_REQUIRE(ngroup >= 1 && ngroup <= kGroupedMax, GA_ERR_INVALID,
+             "a grouped launch takes 1 to %d geometries", kGroupedMax);
+  f16_group grp{};
+  grp.ngroup = ngroup;
+  int64_t begin = 0;
+  for (int i = 0; i < ngroup; ++i) {
+    const ga_plan *p = plans[i];
+    const ga_tuning tu = p->tune;
+    GA_REQUIRE(tu.mfma_path != 1, GA_ERR_UNSUPPORTED,
+               "geometry %d is tuned to the FP32 MFMA path; a grouped launch runs split FP16", i);
+    const int acc_form = f16_accumulation(p, tu, S[i]);
+    GA_REQUIRE(acc_form < 0, GA_ERR_UNSUPPORTED,
+               "geometry %d accumulates in register master sums (lmax %d); a grouped launch "
+               "runs one chain or l-chunks",
+               i, p->lmax);
+    const f32_ws_layout w = f32_workspace(p, S[i]);
+    GA_REQUIRE(acc_form != -2 || (w.p_bytes > 0 && f16_park_fits(p, S[i])), GA_ERR_UNSUPPORTED,
+               "geometry %d: two-chunk accumulation does not fit %d shells", i, S[i]);
+    char *base = static_cast<char *>(ws[i]);
+    int nt = tu.mfma_nt;
+    if (nt != 1 && nt != 2) nt = 2 * S[i] <= 32 ? 1 : 2;
+    int64_t nblk = 0;
+    grp.g[i] = f16_geometry(p, S[i], reinterpret_cast<float *>(base), base + w.b_off,
+                            reinterpret_cast<const int32_t *>(base + w.e_off), nt, acc_form,
+                            reinterpret_cast<float *>(base + w.p_off), &nblk);
+    grp.g[i].blk_begin = int(begin);  // a multiple of 8: XCD mapping as in one launch
+    begin += nblk;
+    GA_REQUIRE(begin < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
+  }
+  hipLaunchKernelGGL(legendre_synth_f16_grouped_kernel<2>, dim3(unsigned(begin)), dim3(256), 0,
+                     stream, grp);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
@@ -798,13 +945,10 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
   if (tu.mfma_path == 1) return launch_legendre_synth_mfma(p, tu, S, alm, F, stream);
   const f32_ws_layout w = f32_workspace(p, S);
   const char *base = static_cast<const char *>(ws);
-  const int acc_form = f16_accumulation(p, tu);
-  GA_REQUIRE(acc_form > -2 || w.p_bytes > 0, GA_ERR_WORKSPACE,
-             "the workspace was sized for another accumulation form; size it again");
-  return launch_legendre_contract_f16(p, S, F, base + w.b_off,
-                                      reinterpret_cast<const int32_t *>(base + w.e_off), stream,
-                                      tu.mfma_nt, acc_form,
-                                      const_cast<char *>(base) + w.p_off);
+  const int acc_form = f16_accumulation(p, tu, S);
+  return launch_legendre_contract_f16(
+      p, S, F, base + w.b_off, reinterpret_cast<const int32_t *>(base + w.e_off), stream,
+      tu.mfma_nt, acc_form, reinterpret_cast<float *>(const_cast<char *>(base) + w.p_off));
 }
 
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,

@@ -349,6 +349,11 @@ class DeviceExecutor:
     def close_lane(self, key) -> None:
         ln = self.lanes.pop(key)
         ln.stream.synchronize()
+        del ln
+        # a lane's buffers are the largest blocks: hand them back to the
+        # device rather than keep them cached, so that lanes of other sizes
+        # opened later do not fail on a fragmented cache
+        self.T.cuda.empty_cache()
 
     # requests
     def open_request(self, req: SimRequest):

@@ -5,11 +5,17 @@ Maps are grouped by geometry.  Each group is a batched engine
 (``LognormalShells`` with ncorr = 0, so every map has its own spectrum and
 its own normals, keyed by (seed, group, map)); the groups' kernel sequences
 (iternorm -> alm -> MFMA Legendre -> ring FFT + lognormal -> analysis ->
-spin-2 shear) are captured once into a HIP graph, one concurrent branch per
-geometry, so a replay submits the whole uneven batch with one host call and
-the small geometries fill the device while the large ones run.  The Legendre
-tile configuration of every geometry is chosen by the autotuner before the
-capture (``glass_amd.autotune``).
+spin-2 shear) are captured once into a HIP graph through the library's graph
+entry points (``ga_graph_begin`` / ``ga_graph_end``), so a replay submits the
+whole uneven batch with one host call.  The dominant stage, the split-FP16
+Legendre contraction, runs for all geometries in ONE grouped launch
+(``ga_legendre_contract_grouped_f32``: the largest geometry's blocks first,
+the small geometries' blocks filling the device as it drains); the stages
+before and after it run as one concurrent graph branch per geometry.  The
+tile configuration of every geometry (one or two 32-column tiles per wave)
+is chosen by the autotuner before the capture (``glass_amd.autotune``).
+``grouped=False`` keeps one contraction launch per geometry inside its
+branch.
 
 The synthetic workload: 128 maps, nside uniform over {256, 512, 1024, 2048},
 lmax = 2 nside - 1, C_l = 1e-4 (l+1)^t exp(-(l/1500)^2) with a tilt t uniform
@@ -24,6 +30,9 @@ import numpy as np
 
 from . import _lib
 from .batch import LognormalShells
+
+
+GROUP_MAX = 8  # geometries per grouped contraction launch
 
 
 @dataclasses.dataclass
@@ -58,7 +67,7 @@ def independent_gls(spectra):
 
 class UnevenBatch:
     def __init__(self, specs, *, shift: float = 1.0, shear: bool = True, seed: int = 3,
-                 tune: bool = True):
+                 tune: bool = True, grouped: bool = True):
         T = _lib.torch()
         _lib.require_gpu()
         self.specs = list(specs)
@@ -76,12 +85,15 @@ class UnevenBatch:
             gls = independent_gls([map_spectrum(lmax, sp.tilt) for sp in sps])
             self.engines[key] = LognormalShells(gls, nside, ncorr=0, shift=shift,
                                                 seed=seed * 1000 + gi, lmax=lmax, shear=shear)
+        self.grouped = bool(grouped)
         if tune:
             from . import autotune
 
+            cands = autotune.GROUPED_CANDIDATES if self.grouped else autotune.CANDIDATES
             for key, eng in self.engines.items():
-                self.tuning[key] = autotune.tune_legendre(eng)
+                self.tuning[key] = autotune.tune_legendre(eng, candidates=cands)
         self.streams = {key: T.cuda.Stream() for key in self.keys}
+        self.capture_stream = T.cuda.Stream()
         self.graph = None
         self.nmaps = len(self.specs)
 
@@ -89,35 +101,57 @@ class UnevenBatch:
         for key in self.keys:
             self.engines[key].run()
 
-    def _branches(self):
+    def _fork(self, work):
+        """work(engine) on every geometry's branch stream, joined back."""
         T = _lib.torch()
         main = T.cuda.current_stream()
         for key in self.keys:
             s = self.streams[key]
             s.wait_stream(main)
             with T.cuda.stream(s):
-                self.engines[key].run()
+                work(self.engines[key])
         for key in self.keys:
             main.wait_stream(self.streams[key])
 
+    def _branches(self):
+        if not self.grouped:
+            self._fork(lambda eng: eng.run())
+            return
+
+        def before(eng):
+            eng.stage_fields()
+            eng.stage_operands()
+
+        def after(eng):
+            eng.stage_rings()
+            eng.stage_shear()
+
+        self._fork(before)
+        for i in range(0, len(self.keys), GROUP_MAX):  # the library takes 8 per launch
+            engs = [self.engines[k] for k in self.keys[i:i + GROUP_MAX]]
+            _lib.contract_grouped([e.plan for e in engs], [e.S for e in engs],
+                                  [e.F for e in engs])
+        self._fork(after)
+
     def capture(self):
-        """Record the whole batch into one HIP graph (all buffers are already
-        allocated; the library's entry points neither allocate nor set memory
-        with memset, so they are capturable)."""
+        """Record the whole batch into one HIP graph through the library's
+        graph entry points (all buffers are already allocated; the library's
+        entry points neither allocate nor set memory with memset)."""
         T = _lib.torch()
-        # one eager pass on the branch streams first (lazy tables, allocator state)
-        self._branches()
-        T.cuda.synchronize()
-        g = T.cuda.CUDAGraph()
-        with T.cuda.graph(g):
+        cs = self.capture_stream
+        cs.wait_stream(T.cuda.current_stream())
+        with T.cuda.stream(cs):
+            # one eager pass first (code objects, lazily built tables)
             self._branches()
-        self.graph = g
-        return g
+            cs.synchronize()
+            self.graph = _lib.Graph().capture(self._branches)
+        T.cuda.current_stream().wait_stream(cs)
+        return self.graph
 
     def replay(self):
         if self.graph is None:
             self.capture()
-        self.graph.replay()
+        self.graph.launch()
 
     def check(self):
         for eng in self.engines.values():

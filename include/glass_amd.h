@@ -73,11 +73,13 @@ enum ga_tune_key {
   GA_TUNE_F16_FLUSH = 7   /* split-FP16 synthesis accumulation (shorter FP32 rounding
                              chains): n > 0 = every n slabs of 32 l rows the MFMA
                              accumulators are added into FP32 master sums in
-                             registers (one column tile per wave); -k (k = 2..8) =
-                             the l-range cut into k chunks, the first k - 1 parked
-                             in the workspace and added back at the end (the
-                             workspace grows: size it under the same setting);
-                             -1 = one chain; 0 = automatic (8 when lmax > 4096,
+                             registers (one column tile per wave); -2 = the l-range
+                             cut in two chunks, the first parked in the ring buffer
+                             and added back at the end (the workspace grows by
+                             (lmax + 1) 2 nshell floats: size it under the same
+                             setting; needs 64 ring rows of the batch within
+                             4 GiB); -1 = one chain; 0 = automatic (8 when
+                             lmax > 4096 or the two chunks do not fit the batch,
                              else -2) */
 };
 int ga_set_tuning(int32_t key, int32_t value);
@@ -171,6 +173,40 @@ int ga_legendre_prepare_colmax_batch_f32(const ga_plan *plan, int32_t nshell, co
                                          size_t ws_bytes, void *stream);
 int ga_ring_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *F, float *maps,
                             const ga_epilogue *epi, void *stream);
+
+/* The split-FP16 contraction of an uneven batch in ONE launch: ngroup
+ * (1..8) geometries, each a plan, its shell count nshell[i] and its FP32
+ * workspace (ws_bytes[i] >= ga_workspace_size(plans[i], nshell[i], 4)) that
+ * ga_legendre_prepare_batch_f32 has prepared for the same shells.  The
+ * blocks of all geometries share one grid (list the largest first: its
+ * blocks are dispatched first and the small geometries fill the device as
+ * it drains); each geometry runs on its own plan's tiles and accumulation
+ * form and writes its ring buffer F at the start of its workspace, exactly
+ * as ga_legendre_contract_batch_f32 would.  Rejected (GA_ERR_UNSUPPORTED):
+ * a plan tuned to the FP32 MFMA path or accumulating in register master
+ * sums (lmax > 4096 by default).  The batched form of many alm2map calls of
+ * different nside (glass/healpix.py:38-78 via glass/fields.py:429). */
+int ga_legendre_contract_grouped_f32(int32_t ngroup, const ga_plan *const *plans,
+                                     const int32_t *nshell, void *const *workspace,
+                                     const size_t *ws_bytes, void *stream);
+
+/* Graph capture for C hosts: every entry point of this library that takes a
+ * workspace allocates nothing and sets no memory with memset, so a sequence
+ * of calls on one stream can be recorded once and replayed with one host
+ * call.  ga_graph_begin starts capturing `stream` (a created stream, not the
+ * null stream; thread-local capture mode); ga_graph_end stops it and
+ * instantiates the recorded work; ga_graph_launch replays it on a stream;
+ * ga_graph_destroy frees it.  Plans must exist, and spin tables be prepared
+ * (ga_plan_prepare_spin), before the capture begins; the buffers the
+ * recorded calls name must outlive the graph.  If a call fails during the
+ * capture, ga_graph_end still ends it and returns the error.  No counterpart
+ * in the reference (which calls healpy once per map): the graph and
+ * batch-schedule entry points of the batched path. */
+typedef struct ga_graph ga_graph;
+int ga_graph_begin(void *stream);
+int ga_graph_end(void *stream, ga_graph **out);
+int ga_graph_launch(const ga_graph *graph, void *stream);
+int ga_graph_destroy(ga_graph *graph);
 
 /* FP64 scalar analysis with healpy's Jacobi refinement (niter passes,
  * alm += A(map - S(alm))), uniform pixel weights 4 pi / npix.

@@ -38,9 +38,12 @@ def test_uneven_specs_workload():
     assert uneven_specs() == sp
 
 
-def test_graph_replays_match_eager_and_fp64():
+@pytest.mark.parametrize("grouped", [True, False])
+def test_graph_replays_match_eager_and_fp64(grouped):
+    """Graph replays (grouped: one contraction launch for all geometries)
+    equal the eager run (one contraction launch per geometry) bit for bit."""
     specs = uneven_specs(n=12, seed=3, nsides=(16, 32, 64))
-    ub = UnevenBatch(specs, shear=True, tune=True)
+    ub = UnevenBatch(specs, shear=True, tune=True, grouped=grouped)
     ub.run_eager()
     torch.cuda.synchronize()
     ub.check()
