@@ -54,17 +54,17 @@ struct ga_tuning {
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
   int f16_flush = 0;    // split-FP16 synthesis accumulation: n > 0 master sums in registers
-                        // flushed every n slabs (one column tile), -2 two l-chunks (the
-                        // first parked in the ring buffer), -1 one chain, 0 auto
-                        // (kAutoFlushSlabs when lmax > kAutoFlushLmax or the parking does
-                        // not fit the batch, else kAutoChunks)
+                        // flushed every n slabs (one column tile), -2 / -3 two l-chunks
+                        // (the first parked in the ring buffer / in a region of its own,
+                        // faster, as large as the ring buffer), -1 one chain, 0 auto
+                        // (kAutoFlushSlabs when lmax > kAutoFlushLmax, else kAutoChunks)
 };
 // lmax above which the split-FP16 synthesis flushes its MFMA accumulators into
 // register master sums by default, the flush period in slabs, and the default
 // accumulation below that lmax
 constexpr int kAutoFlushLmax = 4096;
 constexpr int kAutoFlushSlabs = 8;
-constexpr int kAutoChunks = -2;
+constexpr int kAutoChunks = -3;
 ga_tuning &tuning();  // the process default
 
 // l rows per slab of the FP16-split MFMA Legendre stage (K = 16 per parity)

@@ -25,11 +25,12 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
                                 const double *csc = nullptr);
 // nt: column tiles per wave (0 automatic, 1, 2, 4 = wave pairs); flush: the
 // accumulation form (f16_accumulation: -1 one chain, n > 0 register master
-// sums every n slabs, -2 two l-chunks, the first parked in F, the O half of
-// its equator row in eqo: (lmax + 1) 2 S floats)
+// sums every n slabs, -2 two l-chunks with the first parked in F and the O
+// half of its equator row at park ((lmax + 1) 2 S floats), -3 two l-chunks
+// with the first parked at park, one 16 nt KiB slot per wave of the grid)
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s, int nt, int flush,
-                                 float *eqo);
+                                 void *park);
 // the batched Legendre stage on the path chosen by the tuning set tu (a
 // snapshot of the plan's, the same for the preparation and the contraction
 // of one synthesis); ws is the FP32 workspace with F at its start

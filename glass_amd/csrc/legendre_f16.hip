@@ -371,6 +371,41 @@ __device__ __forceinline__ void park_tile(f32x16 (&acc)[2][2][NT], int m, int lm
   }
 }
 
+// Two-chunk parking in a region of its own: the wave's accumulators as 16 NT
+// float4 per lane, lane-major (every store and load a whole 1 KiB per wave),
+// in the wave's slot of the region (one slot per wave of the grid, 16 NT
+// KiB).  Faster than parking in F (whose 32-byte column groups make every
+// store and load a scatter), at the cost of a region as large as F.
+template <int NT, bool LOAD>
+__device__ __forceinline__ void park_lanes(f32x16 (&acc)[2][2][NT], float4 *__restrict__ slot) {
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) {
+        float4 *const q4 = slot + ((p * 2 + t) * NT + ct) * 4 * 64;
+        if (LOAD) {
+          float4 o[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) o[q] = q4[q * 64];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            acc[p][t][ct][4 * q] += o[q].x;
+            acc[p][t][ct][4 * q + 1] += o[q].y;
+            acc[p][t][ct][4 * q + 2] += o[q].z;
+            acc[p][t][ct][4 * q + 3] += o[q].w;
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            q4[q * 64] = make_float4(acc[p][t][ct][4 * q], acc[p][t][ct][4 * q + 1],
+                                     acc[p][t][ct][4 * q + 2], acc[p][t][ct][4 * q + 3]);
+        }
+      }
+}
+
 // One geometry of the split-FP16 contraction: plan tables, the prepared B
 // operands and column exponents, the ring buffer F and the parking space of
 // the equator row's O half (two-chunk form).
@@ -381,7 +416,9 @@ struct f16_geom {
   const uint4 *B;
   const int32_t *colexp;
   float *F;
-  float *eqo;
+  float *eqo;     // two chunks parked in F: the equator row's O half
+  float4 *part;   // two chunks parked in a region of their own (sep)
+  int sep;
   int lmax, npairs, nrings, ncols, nct, nrg, ncg, lam_exp, flush, nchunk, nt, blk_begin;
 };
 
@@ -461,7 +498,10 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
 // The body works on one geometry, described by an f16_geom, for block b of
 // that geometry's grid; the plain kernel runs one geometry, the grouped
 // kernel below several in one grid.
-template <int NT, bool FLUSH>
+// SEP: the two-chunk form parks in a region of its own (park_lanes) instead
+// of in F (park_tile); one variant per kernel keeps the register allocation
+// of each within the 256 registers of two waves per SIMD.
+template <int NT, bool FLUSH, bool SEP>
 __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) {
   const int lmax = g.lmax, npairs = g.npairs, nrings = g.nrings, ncols = g.ncols, nct = g.nct,
             nrg = g.nrg, ncg = g.ncg, lam_exp = g.lam_exp, flush = g.flush, nchunk = g.nchunk;
@@ -594,7 +634,8 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   const bool two = !FLUSH && nchunk == 2 && nsl - s >= 8;
   run_slabs(two ? s + (nsl - s) / 2 : nsl);
   if (two) {
-    park_tile<NT, false>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
+    if (SEP) park_lanes<NT, false>(acc, g.part + (int64_t(b) * 4 + wave) * (16 * NT * 64) + lane);
+    else park_tile<NT, false>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -602,7 +643,8 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
     run_slabs(nsl);
-    park_tile<NT, true>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
+    if (SEP) park_lanes<NT, true>(acc, g.part + (int64_t(b) * 4 + wave) * (16 * NT * 64) + lane);
+    else park_tile<NT, true>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
   }
   if (FLUSH) {
 #pragma unroll
@@ -616,9 +658,9 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
                  colexp, F, lam_exp);
 }
 
-template <int NT, int MINW, bool FLUSH>
+template <int NT, int MINW, bool FLUSH, bool SEP>
 __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(const f16_geom g) {
-  contract_blocks<NT, FLUSH>(g, int(blockIdx.x));
+  contract_blocks<NT, FLUSH, SEP>(g, int(blockIdx.x));
 }
 
 // Several geometries in one grid (an uneven batch): geometry i owns blocks
@@ -627,7 +669,7 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_kernel(const f16
 // CUs as the large ones drain.  The geometry of a block is uniform, and so is
 // its tile count (one or two 32-column tiles per wave, both bodies are
 // compiled in).  One accumulation chain or l-chunks, no register master sums.
-template <int MINW>
+template <int MINW, bool SEP>
 __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_grouped_kernel(
     const f16_group grp) {
   const int bx = int(blockIdx.x);
@@ -640,8 +682,8 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_f16_grouped_kernel(
   for (int i = 1; i < kGroupedMax; ++i)
     if (i == gi) g = grp.g[i];
   const int b = bx - g.blk_begin;
-  if (g.nt == 2) contract_blocks<2, false>(g, b);
-  else contract_blocks<1, false>(g, b);
+  if (g.nt == 2) contract_blocks<2, false, SEP>(g, b);
+  else contract_blocks<1, false, SEP>(g, b);
 }
 
 // Producer / consumer form.  In the single-wave form every wave runs the
@@ -818,7 +860,7 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 // the f16_geom of plan p for S shells (flush resolved, nt 1 or 2) and its
 // block count
 static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
-                             const int32_t *colexp, int nt, int flush, float *eqo,
+                             const int32_t *colexp, int nt, int flush, void *park,
                              int64_t *nblk) {
   f16_geom g{};
   g.z = p->d_z;
@@ -831,7 +873,9 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.B = reinterpret_cast<const uint4 *>(bws);
   g.colexp = colexp;
   g.F = F;
-  g.eqo = eqo;
+  g.sep = flush == -3;
+  g.eqo = flush == -2 ? static_cast<float *>(park) : nullptr;
+  g.part = flush == -3 ? static_cast<float4 *>(park) : nullptr;
   g.lmax = p->lmax;
   g.npairs = int(p->npairs);
   g.nrings = int(p->nrings);
@@ -841,7 +885,7 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.ncg = (g.ncols + 32 * nt - 1) / (32 * nt);
   g.lam_exp = p->lam_exp16;
   g.flush = flush;
-  g.nchunk = flush == -2 ? 2 : 1;
+  g.nchunk = (flush == -2 || flush == -3) ? 2 : 1;
   g.nt = nt;
   g.blk_begin = 0;
   *nblk = int64_t(8) * ((p->lmax + 1 + 7) / 8) * g.nrg * g.ncg;
@@ -850,15 +894,17 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t stream, int nt, int flush,
-                                 float *eqo) {
+                                 void *park) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
   // flush (resolved): n > 0 master sums in registers (they take the registers
-  // of a second tile); -2: two l-chunks, the first parked in F
-  GA_REQUIRE(flush == -1 || flush == -2 || flush > 0, GA_ERR_INVALID,
-             "accumulation form %d out of range", flush);
-  GA_REQUIRE(flush != -2 || (eqo && f16_park_fits(p, S)), GA_ERR_UNSUPPORTED,
+  // of a second tile); -2 / -3: two l-chunks, the first parked in F / in a
+  // region of its own
+  GA_REQUIRE(flush >= -3 && flush != 0, GA_ERR_INVALID, "accumulation form %d out of range",
+             flush);
+  GA_REQUIRE(flush != -3 || park, GA_ERR_INVALID, "two-chunk parking needs its region");
+  GA_REQUIRE(flush != -2 || (park && f16_park_fits(p, S)), GA_ERR_UNSUPPORTED,
              "two-chunk accumulation needs 64 ring rows of %d shells within 4 GiB; "
              "use one chain (-1) or register master sums (n > 0)", S);
   if (flush > 0) nt = 1;
@@ -878,14 +924,16 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
     return GA_OK;
   }
   int64_t nblk2 = 0;
-  const f16_geom g = f16_geometry(p, S, F, bws, colexp, nt, flush, eqo, &nblk2);
+  const f16_geom g = f16_geometry(p, S, F, bws, colexp, nt, flush, park, &nblk2);
   GA_REQUIRE(nblk2 < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
-#define GA_F16_LAUNCH(NT_, MW_, FL_)                                                            \
-  hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_, FL_>), dim3(unsigned(nblk2)),        \
+#define GA_F16_LAUNCH(NT_, MW_, FL_, SEP_)                                                      \
+  hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_, FL_, SEP_>), dim3(unsigned(nblk2)),  \
                      dim3(256), 0, stream, g)
-  if (flush > 0) GA_F16_LAUNCH(1, 2, true);
-  else if (nt == 1) GA_F16_LAUNCH(1, 2, false);
-  else GA_F16_LAUNCH(2, 2, false);
+  if (flush > 0) GA_F16_LAUNCH(1, 2, true, false);
+  else if (nt == 1 && flush == -3) GA_F16_LAUNCH(1, 2, false, true);
+  else if (nt == 1) GA_F16_LAUNCH(1, 2, false, false);
+  else if (flush == -3) GA_F16_LAUNCH(2, 2, false, true);
+  else GA_F16_LAUNCH(2, 2, false, false);
 #undef GA_F16_LAUNCH
   GA_LAUNCH_CHECK();
   return GA_OK;
@@ -917,13 +965,25 @@ int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, co
     int64_t nblk = 0;
     grp.g[i] = f16_geometry(p, S[i], reinterpret_cast<float *>(base), base + w.b_off,
                             reinterpret_cast<const int32_t *>(base + w.e_off), nt, acc_form,
-                            reinterpret_cast<float *>(base + w.p_off), &nblk);
+                            base + w.p_off, &nblk);
     grp.g[i].blk_begin = int(begin);  // a multiple of 8: XCD mapping as in one launch
     begin += nblk;
     GA_REQUIRE(begin < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
   }
-  hipLaunchKernelGGL(legendre_synth_f16_grouped_kernel<2>, dim3(unsigned(begin)), dim3(256), 0,
-                     stream, grp);
+  // one parking variant per grid: the geometries' two-chunk forms must agree
+  bool sep = false, inf = false;
+  for (int i = 0; i < ngroup; ++i) {
+    if (grp.g[i].nchunk != 2) continue;
+    (grp.g[i].sep ? sep : inf) = true;
+  }
+  GA_REQUIRE(!(sep && inf), GA_ERR_UNSUPPORTED,
+             "a grouped launch takes one two-chunk parking form (-2 or -3), not both");
+  if (sep)
+    hipLaunchKernelGGL((legendre_synth_f16_grouped_kernel<2, true>), dim3(unsigned(begin)),
+                       dim3(256), 0, stream, grp);
+  else
+    hipLaunchKernelGGL((legendre_synth_f16_grouped_kernel<2, false>), dim3(unsigned(begin)),
+                       dim3(256), 0, stream, grp);
   GA_LAUNCH_CHECK();
   return GA_OK;
 }
@@ -948,7 +1008,7 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
   const int acc_form = f16_accumulation(p, tu, S);
   return launch_legendre_contract_f16(
       p, S, F, base + w.b_off, reinterpret_cast<const int32_t *>(base + w.e_off), stream,
-      tu.mfma_nt, acc_form, reinterpret_cast<float *>(const_cast<char *>(base) + w.p_off));
+      tu.mfma_nt, acc_form, const_cast<char *>(base) + w.p_off);
 }
 
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,

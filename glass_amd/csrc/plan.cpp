@@ -214,7 +214,7 @@ int tuning_set(ga_tuning &t, int32_t key, int32_t value) {
       t.anal_path = value;
       return GA_OK;
     case GA_TUNE_F16_FLUSH:
-      GA_REQUIRE(value >= -2 && value <= 4096, GA_ERR_INVALID, "f16 flush out of range");
+      GA_REQUIRE(value >= -3 && value <= 4096, GA_ERR_INVALID, "f16 flush out of range");
       t.f16_flush = value;
       return GA_OK;
     default:
@@ -549,6 +549,14 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
     // two chunks park the first in the ring buffer itself; only the O half
     // of the equator row needs a place of its own: (lmax + 1) x 2 nbatch
     w.p_bytes = up(size_t(p->lmax + 1) * size_t(2 * nbatch) * 4);
+  } else if (acc_form == -3) {
+    // two chunks park the first in a slot per wave of the two-tile launch
+    // grid (8 m-groups x ceil((lmax + 1) / 8) x ring groups x 64-column
+    // groups, 4 waves each): 32 float4 per lane; the one-tile grid needs at
+    // most as much
+    const size_t nm8 = size_t(p->lmax + 1 + 7) / 8, nrg = size_t(p->npairs + 255) / 256;
+    const size_t ncg = size_t(2 * nbatch + 63) / 64;
+    w.p_bytes = up(8 * nm8 * nrg * ncg * 4 * 64 * 32 * 16);
   }
   w.total = w.p_off + w.p_bytes;
   return w;
@@ -560,8 +568,8 @@ bool f16_park_fits(const ga_plan *p, int nbatch) {
 }
 
 int f16_accumulation(const ga_plan *p, const ga_tuning &tu, int nbatch) {
+  (void)nbatch;
   if (tu.f16_flush != 0) return tu.f16_flush;
-  if (p->lmax > kAutoFlushLmax || !f16_park_fits(p, nbatch)) return kAutoFlushSlabs;
-  return kAutoChunks;
+  return p->lmax > kAutoFlushLmax ? kAutoFlushSlabs : kAutoChunks;
 }
 }  // namespace ga

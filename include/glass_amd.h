@@ -73,14 +73,15 @@ enum ga_tune_key {
   GA_TUNE_F16_FLUSH = 7   /* split-FP16 synthesis accumulation (shorter FP32 rounding
                              chains): n > 0 = every n slabs of 32 l rows the MFMA
                              accumulators are added into FP32 master sums in
-                             registers (one column tile per wave); -2 = the l-range
-                             cut in two chunks, the first parked in the ring buffer
-                             and added back at the end (the workspace grows by
-                             (lmax + 1) 2 nshell floats: size it under the same
-                             setting; needs 64 ring rows of the batch within
-                             4 GiB); -1 = one chain; 0 = automatic (8 when
-                             lmax > 4096 or the two chunks do not fit the batch,
-                             else -2) */
+                             registers (one column tile per wave); -3 / -2 = the
+                             l-range cut in two chunks, the first parked and added
+                             back at the end: -3 in a region of the workspace as
+                             large as the ring buffer (coalesced, fastest), -2 in the
+                             ring buffer itself (the workspace grows by only
+                             (lmax + 1) 2 nshell floats; needs 64 ring rows of the
+                             batch within 4 GiB); size the workspace under the same
+                             setting; -1 = one chain; 0 = automatic (8 when
+                             lmax > 4096, else -3) */
 };
 int ga_set_tuning(int32_t key, int32_t value);
 int ga_get_tuning(int32_t key, int32_t *value);

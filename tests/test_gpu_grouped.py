@@ -43,6 +43,12 @@ def _ring_bytes(eng):
 def test_grouped_contraction_equals_single_launches():
     geoms = [(128, 255, 20, 1, 2, -2), (64, 127, 9, 2, 1, -1), (32, 63, 40, 3, 2, -2),
              (16, 40, 3, 4, 1, -2), (8, 30, 33, 5, 2, -1)]
+    run_grouped(geoms)
+    # the other parking form, in its own launch (one form per grid)
+    run_grouped([(128, 255, 20, 1, 2, -3), (32, 63, 40, 3, 1, -3), (16, 40, 3, 4, 2, -1)])
+
+
+def run_grouped(geoms):
     engs = [_engine(*g) for g in geoms]
     torch.cuda.synchronize()
     refs = []
@@ -77,6 +83,12 @@ def test_grouped_contraction_rejects():
     e2.plan.set_tuning(_lib.TUNE_MFMA_PATH, 2)
     e2.plan.set_tuning(_lib.TUNE_F16_FLUSH, 8)
     with pytest.raises(NotImplementedError, match="master sums"):
+        _lib.contract_grouped([e1.plan, e2.plan], [4, 4], [e1.F, e2.F])
+    e2.plan.set_tuning(_lib.TUNE_F16_FLUSH, -3)
+    e1.plan.set_tuning(_lib.TUNE_F16_FLUSH, -2)
+    e2.F = _lib.workspace(e2.plan.workspace_bytes(4, 4))
+    e1.F = _lib.workspace(e1.plan.workspace_bytes(4, 4))
+    with pytest.raises(NotImplementedError, match="one two-chunk parking form"):
         _lib.contract_grouped([e1.plan, e2.plan], [4, 4], [e1.F, e2.F])
     e2.plan.set_tuning(_lib.TUNE_F16_FLUSH, -1)
     with pytest.raises(RuntimeError, match="workspace"):

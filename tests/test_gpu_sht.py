@@ -88,7 +88,7 @@ def test_alm2map_fp32_mfma_lmax8192(nside):
         assert rms < 1e-6 and e < 1e-5, (s, rms, e)
 
 
-@pytest.mark.parametrize("flush", [4, -2])
+@pytest.mark.parametrize("flush", [4, -2, -3])
 def test_fp32_accumulation_forms_are_consistent(flush):
     """The accumulation forms of the split-FP16 synthesis (register master sums
     every `flush` slabs, or the l-range in two chunks, the first parked in the
