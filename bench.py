@@ -55,6 +55,8 @@ def parse():
                     help="seconds of Poisson arrivals for the serving-loop section (0 = skip)")
     ap.add_argument("--uneven-maps", type=int, default=128,
                     help="maps in the uneven-batch section (0 = skip)")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="skip the kernels of the rows beside the hot path")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB first (a known-size "
                          "kernel that calibrates hardware byte counters)")
@@ -157,6 +159,100 @@ def uneven_section(nmaps: int):
             "tuning": {f"nside{k[0]}": {"path": c["path"], "nt": c["nt"]}
                        for k, c in ub.tuning.items()},
             "accuracy_vs_fp64_all_maps": errs}
+
+
+def extras_section(nside: int):
+    """The kernels of the rows beside the hot path, each timed with HIP events
+    on the launch stream against its own bound, on maps of the headline
+    geometry (nside 2048):
+
+    * multi-plane convergence update, float32 maps: reads kappa1, kappa2,
+      delta2, writes kappa3 (16 bytes per pixel), HBM roofline;
+    * positions from a lognormal density map, device Philox counts and
+      positions, ~1e8 points: expected counts + Poisson counts (reads the map,
+      writes int64 counts), the offset scan, the placement (reads counts,
+      writes two float64 per point); points/s;
+    * the pixel window of the geometry (spin 0 and 2, up to l = 4 nside);
+    * the Gaussian-spectra solver on the 64 lognormal spectra of the headline
+      batch (host loop over device Gauss-Legendre transforms)."""
+    import torch
+
+    from glass_amd import _lib, healpix, points
+    from glass_amd.batch import shell_spectra
+    from glass_amd import grf
+
+    out = {}
+    npix = 12 * nside * nside
+    st = _lib.stream_ptr()
+
+    def timed(fn, reps=5):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    k1 = torch.randn(npix, device="cuda", generator=g)
+    k2 = torch.randn(npix, device="cuda", generator=g)
+    d2 = torch.randn(npix, device="cuda", generator=g)
+    k3 = torch.empty_like(k1)
+    ms = timed(lambda: _lib.call("ga_multi_plane_update", npix, 4, _lib.ptr(k1), _lib.ptr(k2),
+                                 _lib.ptr(d2), 0.3, 0.01, _lib.ptr(k3), st))
+    gbs = 16.0 * npix / (ms * 1e-3) / 1e9
+    out["multi_plane_update"] = {"nside": nside, "ms": ms, "bytes": 16 * npix, "GB_per_s": gbs,
+                                 "frac_of_8TBs": gbs / 8000.0}
+    del k1, k2, k3
+
+    delta = torch.expm1(0.5 * d2.double() - 0.125)  # lognormal contrast, mean ~0
+    nb = _lib.ctypes.c_size_t()
+    _lib.call("ga_points_workspace_size", npix, _lib.ctypes.byref(nb))
+    ws = _lib.workspace(nb.value)
+    counts = torch.empty(npix, dtype=torch.int64, device="cuda")
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ngal = 1e8 / points.ARCMIN2_SPHERE
+    scale = points.ARCMIN2_SPHERE / npix * ngal
+
+    def do_counts():
+        _lib.call("ga_points_counts", npix, 8, _lib.ptr(delta), 1, 1.0, 0, scale, None, 7, 0,
+                  None, _lib.ptr(counts), _lib.ptr(total), _lib.ptr(ws), ws.numel(), st)
+
+    ms_counts = timed(do_counts)
+    ntot = int(total.item())
+    lon = torch.empty(ntot, dtype=torch.float64, device="cuda")
+    lat = torch.empty(ntot, dtype=torch.float64, device="cuda")
+    ms_pos = timed(lambda: _lib.call("ga_points_positions", nside, _lib.ptr(counts), 7, 0,
+                                     _lib.ptr(lon), _lib.ptr(lat), _lib.ptr(ws), ws.numel(), st))
+    out["positions_from_delta"] = {
+        "nside": nside, "points": ntot, "counts_ms": ms_counts, "positions_ms": ms_pos,
+        "points_per_s": ntot / ((ms_counts + ms_pos) * 1e-3),
+        "positions_GB_per_s": (16.0 * ntot + 8.0 * npix) / (ms_pos * 1e-3) / 1e9}
+    del lon, lat, counts, delta, d2, ws
+
+    healpix._pixwin_cache.clear()
+    t0 = time.perf_counter()
+    healpix.pixwin(nside, lmax=4 * nside, pol=True)
+    out["pixwin"] = {"nside": nside, "lmax": 4 * nside, "pol": True,
+                     "s": time.perf_counter() - t0}
+
+    gls = shell_spectra(64, 2 * nside, amp=lambda i: 1.0 + 3.0 * i / 63, rho=(0.6, 0.3, 0.1),
+                        scale=1500.0)
+    fields = [grf.Lognormal(1.0) for _ in range(64)]
+    import glass_amd
+
+    t0 = time.perf_counter()
+    sol = glass_amd.solve_gaussian_spectra(fields, gls)
+    out["solve_gaussian_spectra"] = {"spectra": len(sol), "lmax": 2 * nside,
+                                     "s": time.perf_counter() - t0}
+    torch.cuda.empty_cache()
+    return out
 
 
 def accuracy_all_shells(eng, S: int):
@@ -316,6 +412,13 @@ def main():
             out["serving"] = serving_section(args.serve_seconds)
         except Exception as e:
             out["serving"] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
+    if args.extras:
+        print("[bench] extras", file=sys.stderr, flush=True)
+        try:
+            out["extras"] = extras_section(nside)
+        except Exception as e:
+            out["extras"] = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

@@ -116,7 +116,12 @@ extern "C" int ga_alm2map_batch_f32_noalloc(const ga_plan *p, int32_t S, const f
                                             float *maps, const ga_epilogue *epi, void *stream) {
   GA_PLAN_OK(p);
   GA_REQUIRE(S >= 1 && alm && maps, GA_ERR_INVALID, "invalid argument");
-  const size_t need = ring_bytes(p, S, 4);
+  // the compact two-chunk form (first chunk parked in the ring buffer) in
+  // place of the one with a region of its own: bit-identical results, and the
+  // scratch holds twice the batch
+  ga_tuning tu = p->tune;
+  if (f16_accumulation(p, tu, S) == -3) tu.f16_flush = -2;
+  const size_t need = f32_workspace(p, S, tu).total;
   GA_REQUIRE(need <= kNoAllocScratchBytes, GA_ERR_UNSUPPORTED,
              "nside %d, lmax %d, %d maps need %zu bytes of scratch; the fixed scratch holds %zu",
              p->nside, p->lmax, S, need, kNoAllocScratchBytes);
@@ -132,7 +137,7 @@ extern "C" int ga_alm2map_batch_f32_noalloc(const ga_plan *p, int32_t S, const f
   // preparation launch, every other byte they read was written earlier in
   // the same call
   hipStream_t s = (hipStream_t)stream;
-  rc = launch_legendre_synth_batch(p, S, alm, ws, s);
+  rc = launch_legendre_synth_batch(p, tu, S, alm, ws, s);
   if (rc) return rc;
   return launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)ws, maps, epi, s);
 }

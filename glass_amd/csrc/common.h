@@ -53,6 +53,7 @@ struct ga_tuning {
   int ring_path = 0;    // FP32 ring synthesis: 0 auto (fused passes, register fold on wide rings), 1 radix-4 stages, 2 fused passes only
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
+  int m_group = 0;      // split-FP16 synthesis: consecutive m per XCD turn, 0 auto (= 1), 1, 4
   int f16_flush = 0;    // split-FP16 synthesis accumulation: n > 0 master sums in registers
                         // flushed every n slabs (one column tile), -2 / -3 two l-chunks
                         // (the first parked in the ring buffer / in a region of its own,
@@ -79,6 +80,8 @@ __host__ __device__ inline int64_t nslab_of(int64_t lmax, int64_t m) { return (l
 struct f32_ws_layout {
   size_t f_bytes, b_off, b_bytes, e_off, p_off, p_bytes, total;
 };
+f32_ws_layout f32_workspace(const ga_plan *p, int nbatch, const ga_tuning &tu);
+// under the plan's current tuning
 f32_ws_layout f32_workspace(const ga_plan *p, int nbatch);
 // the accumulation form the split-FP16 synthesis of plan p runs under tuning
 // set tu: the f16_flush knob with 0 (automatic) resolved

@@ -28,9 +28,10 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 // sums every n slabs, -2 two l-chunks with the first parked in F and the O
 // half of its equator row at park ((lmax + 1) 2 S floats), -3 two l-chunks
 // with the first parked at park, one 16 nt KiB slot per wave of the grid)
+// mgroup: consecutive m per XCD turn (1 or 4)
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s, int nt, int flush,
-                                 void *park);
+                                 void *park, int mgroup);
 // the batched Legendre stage on the path chosen by the tuning set tu (a
 // snapshot of the plan's, the same for the preparation and the contraction
 // of one synthesis); ws is the FP32 workspace with F at its start
@@ -42,6 +43,8 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
                                    const float *alm, void *ws, hipStream_t s);
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
                                 hipStream_t s);
+int launch_legendre_synth_batch(const ga_plan *p, const ga_tuning tu, int S, const float *alm,
+                                void *ws, hipStream_t s);
 // the split-FP16 contraction of up to kGroupedMax geometries (plans[i], S[i]
 // shells, prepared FP32 workspace ws[i]) in one grid, each on its plan's
 // tiles and accumulation form

@@ -420,6 +420,7 @@ struct f16_geom {
   float4 *part;   // two chunks parked in a region of their own (sep)
   int sep;
   int lmax, npairs, nrings, ncols, nct, nrg, ncg, lam_exp, flush, nchunk, nt, blk_begin;
+  int mgroup;  // consecutive m per XCD turn (1 or 4)
 };
 
 struct f16_group {
@@ -521,9 +522,10 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   // l-range first), all (ring group, column group) tiles of one m back to
   // back so they share that m's B slabs in the XCD's L2.
   const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
-  const int m = 8 * (kq / G) + xcd;
+  const int mq = kq / G;  // this XCD's m slot
+  const int m = g.mgroup == 4 ? 32 * (mq >> 2) + 4 * xcd + (mq & 3) : 8 * mq + xcd;
   if (m > lmax) return;
-  const int rem = kq - (kq / G) * G;
+  const int rem = kq - mq * G;
   const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rp0 = rg * 256 + wave * 64;
@@ -861,7 +863,7 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 // block count
 static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
                              const int32_t *colexp, int nt, int flush, void *park,
-                             int64_t *nblk) {
+                             int mgroup, int64_t *nblk) {
   f16_geom g{};
   g.z = p->d_z;
   g.sth = p->d_sth;
@@ -888,13 +890,16 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.nchunk = (flush == -2 || flush == -3) ? 2 : 1;
   g.nt = nt;
   g.blk_begin = 0;
-  *nblk = int64_t(8) * ((p->lmax + 1 + 7) / 8) * g.nrg * g.ncg;
+  g.mgroup = mgroup == 4 ? 4 : 1;
+  // m slots per XCD: enough for m = 0 .. lmax (a multiple of 4 with runs of 4)
+  const int slots = g.mgroup == 4 ? 4 * ((p->lmax + 1 + 31) / 32) : (p->lmax + 1 + 7) / 8;
+  *nblk = int64_t(8) * slots * g.nrg * g.ncg;
   return g;
 }
 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t stream, int nt, int flush,
-                                 void *park) {
+                                 void *park, int mgroup) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
@@ -924,7 +929,7 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
     return GA_OK;
   }
   int64_t nblk2 = 0;
-  const f16_geom g = f16_geometry(p, S, F, bws, colexp, nt, flush, park, &nblk2);
+  const f16_geom g = f16_geometry(p, S, F, bws, colexp, nt, flush, park, mgroup, &nblk2);
   GA_REQUIRE(nblk2 < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
 #define GA_F16_LAUNCH(NT_, MW_, FL_, SEP_)                                                      \
   hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_, FL_, SEP_>), dim3(unsigned(nblk2)),  \
@@ -956,7 +961,7 @@ int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, co
                "geometry %d accumulates in register master sums (lmax %d); a grouped launch "
                "runs one chain or l-chunks",
                i, p->lmax);
-    const f32_ws_layout w = f32_workspace(p, S[i]);
+    const f32_ws_layout w = f32_workspace(p, S[i], tu);
     GA_REQUIRE(acc_form != -2 || (w.p_bytes > 0 && f16_park_fits(p, S[i])), GA_ERR_UNSUPPORTED,
                "geometry %d: two-chunk accumulation does not fit %d shells", i, S[i]);
     char *base = static_cast<char *>(ws[i]);
@@ -965,7 +970,7 @@ int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, co
     int64_t nblk = 0;
     grp.g[i] = f16_geometry(p, S[i], reinterpret_cast<float *>(base), base + w.b_off,
                             reinterpret_cast<const int32_t *>(base + w.e_off), nt, acc_form,
-                            base + w.p_off, &nblk);
+                            base + w.p_off, tu.m_group, &nblk);
     grp.g[i].blk_begin = int(begin);  // a multiple of 8: XCD mapping as in one launch
     begin += nblk;
     GA_REQUIRE(begin < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
@@ -992,7 +997,7 @@ int launch_legendre_prepare_batch(const ga_plan *p, const ga_tuning &tu, int S, 
                                   void *ws, hipStream_t stream, const float *colpart,
                                   int64_t nparts) {
   if (tu.mfma_path == 1) return GA_OK;  // the FP32 path reads the alm directly
-  const f32_ws_layout w = f32_workspace(p, S);
+  const f32_ws_layout w = f32_workspace(p, S, tu);
   char *base = static_cast<char *>(ws);
   return launch_legendre_prepare_f16(p, S, alm, base + w.b_off,
                                      reinterpret_cast<int32_t *>(base + w.e_off), stream, 0,
@@ -1003,17 +1008,21 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
                                    const float *alm, void *ws, hipStream_t stream) {
   float *F = static_cast<float *>(ws);
   if (tu.mfma_path == 1) return launch_legendre_synth_mfma(p, tu, S, alm, F, stream);
-  const f32_ws_layout w = f32_workspace(p, S);
+  const f32_ws_layout w = f32_workspace(p, S, tu);
   const char *base = static_cast<const char *>(ws);
   const int acc_form = f16_accumulation(p, tu, S);
   return launch_legendre_contract_f16(
       p, S, F, base + w.b_off, reinterpret_cast<const int32_t *>(base + w.e_off), stream,
-      tu.mfma_nt, acc_form, const_cast<char *>(base) + w.p_off);
+      tu.mfma_nt, acc_form, const_cast<char *>(base) + w.p_off, tu.m_group);
 }
 
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,
                                 hipStream_t stream) {
-  const ga_tuning tu = p->tune;  // one snapshot for both launches
+  return launch_legendre_synth_batch(p, p->tune, S, alm, ws, stream);  // one snapshot
+}
+
+int launch_legendre_synth_batch(const ga_plan *p, const ga_tuning tu, int S, const float *alm,
+                                void *ws, hipStream_t stream) {
   int rc = launch_legendre_prepare_batch(p, tu, S, alm, ws, stream);
   if (rc) return rc;
   return launch_legendre_contract_batch(p, tu, S, alm, ws, stream);

@@ -213,6 +213,10 @@ int tuning_set(ga_tuning &t, int32_t key, int32_t value) {
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "analysis path must be 0, 1 or 2");
       t.anal_path = value;
       return GA_OK;
+    case GA_TUNE_M_GROUP:
+      GA_REQUIRE(value == 0 || value == 1 || value == 4, GA_ERR_INVALID, "m group must be 0, 1 or 4");
+      t.m_group = value;
+      return GA_OK;
     case GA_TUNE_F16_FLUSH:
       GA_REQUIRE(value >= -3 && value <= 4096, GA_ERR_INVALID, "f16 flush out of range");
       t.f16_flush = value;
@@ -232,6 +236,7 @@ int tuning_get(const ga_tuning &t, int32_t key, int32_t *value) {
     case GA_TUNE_SPIN_PATH: *value = t.spin_path; return GA_OK;
     case GA_TUNE_ANAL_PATH: *value = t.anal_path; return GA_OK;
     case GA_TUNE_F16_FLUSH: *value = t.f16_flush; return GA_OK;
+    case GA_TUNE_M_GROUP: *value = t.m_group; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }
@@ -532,6 +537,10 @@ extern "C" int ga_workspace_size(const ga_plan *p, int32_t nbatch, int32_t real_
 
 namespace ga {
 f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
+  return f32_workspace(p, nbatch, p->tune);
+}
+
+f32_ws_layout f32_workspace(const ga_plan *p, int nbatch, const ga_tuning &tu) {
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   f32_ws_layout w;
   // the ring buffer holds whole groups of 8 shells (the MFMA stage layout)
@@ -544,7 +553,7 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
   w.e_off = w.b_off + w.b_bytes;
   w.p_off = w.e_off + up(nct * 32 * 4 * 2);  // exponents + maxima scratch
   w.p_bytes = 0;
-  const int acc_form = f16_accumulation(p, p->tune, nbatch);
+  const int acc_form = f16_accumulation(p, tu, nbatch);
   if (acc_form == -2) {
     // two chunks park the first in the ring buffer itself; only the O half
     // of the equator row needs a place of its own: (lmax + 1) x 2 nbatch
@@ -554,7 +563,8 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
     // grid (8 m-groups x ceil((lmax + 1) / 8) x ring groups x 64-column
     // groups, 4 waves each): 32 float4 per lane; the one-tile grid needs at
     // most as much
-    const size_t nm8 = size_t(p->lmax + 1 + 7) / 8, nrg = size_t(p->npairs + 255) / 256;
+    // (m slots per XCD: the larger count, that of runs of 4 consecutive m)
+    const size_t nm8 = 4 * (size_t(p->lmax + 1 + 31) / 32), nrg = size_t(p->npairs + 255) / 256;
     const size_t ncg = size_t(2 * nbatch + 63) / 64;
     w.p_bytes = up(8 * nm8 * nrg * ncg * 4 * 64 * 32 * 16);
   }

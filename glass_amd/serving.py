@@ -18,6 +18,15 @@ over time; the server
     launch, and lanes of different geometries overlap on the device;
   * reports per-request latency (arrival to maps ready in HBM).
 
+On the device, every lane buffer and every request's output maps are placed
+in ONE arena of the budget's size, allocated once (``Arena``): lanes from its
+top, requests from its bottom, freed ranges merged with their neighbours.
+Byte accounting alone is not enough at this scale (a request of 80 maps at
+nside 4096 holds 64 GB): with the device allocator's cache, ranges freed by
+requests of other sizes fragment the memory until a large placement fails
+even though the budget has room.  A placement that does not fit the arena
+now makes the request wait, like one that does not fit the budget.
+
 The chain order of a request is kept because all its chunks go through one
 lane stream in order, and the iternorm state is carried between chunks.  The
 device normals are keyed by (request seed, shell, alm index), so a request's
@@ -126,11 +135,18 @@ class Scheduler:
     ``executor`` provides: lane_key(req), lane_capacity(key), lane_bytes(key),
     request_bytes(req), open_lane(key), close_lane(key), open_request(req),
     close_request(state, keep), launch(key, chunks) -> token, poll(token) ->
-    bool, finish(token) (called once when a launch has completed).
+    bool, finish(token) (called once when a launch has completed); and
+    optionally bind(budget_bytes) (called once, before anything is opened).
+    open_lane / open_request may raise MemoryError when the executor cannot
+    place the buffers now: the request then waits (idle lanes are closed
+    first), as when it does not fit the budget.
     """
 
     def __init__(self, executor, budget_bytes: float, depth: int = 2, keep_results: bool = False):
         self.ex = executor
+        bind = getattr(executor, "bind", None)
+        if bind is not None:
+            bind(float(budget_bytes))
         self.budget = float(budget_bytes)
         self.depth = int(depth)
         self.keep = keep_results
@@ -165,17 +181,36 @@ class Scheduler:
                 lane_need = 0 if key in self.lanes else self.ex.lane_bytes(key)
                 if self.used + need + lane_need > self.budget:
                     break
-            self.waiting.popleft()
             if key not in self.lanes:
-                self.ex.open_lane(key)
-                self.lanes[key] = _Lane(key, self.ex.lane_capacity(key), lane_need)
-                self.used += lane_need
-            state = self.ex.open_request(req)
+                if not self._open_lane(key, lane_need):
+                    break
+            try:
+                state = self.ex.open_request(req)
+            except MemoryError:
+                # no room in the executor's arena now: wait for completions
+                # (an idle lane of another geometry is released first)
+                if self._close_idle_lanes(keep=key):
+                    continue
+                break
+            self.waiting.popleft()
             self.lanes[key].active.append(_Active(req, state, need, t_admit=now))
             self.used += need
             self.peak = max(self.peak, self.used)
             did = True
         return did
+
+    def _open_lane(self, key, lane_need) -> bool:
+        for attempt in range(2):
+            try:
+                self.ex.open_lane(key)
+            except MemoryError:
+                if attempt == 0 and self._close_idle_lanes(keep=key):
+                    continue
+                return False
+            self.lanes[key] = _Lane(key, self.ex.lane_capacity(key), lane_need)
+            self.used += lane_need
+            return True
+        return False
 
     def _close_idle_lanes(self, keep) -> bool:
         freed = False
@@ -261,6 +296,9 @@ def run(requests, executor, budget_bytes: float, *, depth: int = 2, keep_results
             progress(now, len(sched.completed), len(reqs))
             next_report += 10.0
     sched.wall = clock() - t0
+    close = getattr(executor, "close", None)
+    if close is not None:
+        close()
     return sched
 
 
@@ -295,6 +333,61 @@ class _ReqState:
     pass
 
 
+class Arena:
+    """Placement of buffers in one device allocation of ``nbytes``: first fit
+    from the bottom (``high=False``) or the top (``high=True``), 4 KiB
+    aligned; released ranges merge with free neighbours."""
+
+    ALIGN = 4096
+
+    def __init__(self, nbytes: int, device="cuda"):
+        T = _lib.torch()
+        self.size = int(nbytes) // self.ALIGN * self.ALIGN
+        self.buf = T.empty(self.size, dtype=T.uint8, device=device)
+        self.free: list = [(0, self.size)]  # (offset, length), sorted by offset
+        self.taken: dict = {}
+
+    def alloc(self, nbytes: int, high: bool = False):
+        n = max(1, -(-int(nbytes) // self.ALIGN)) * self.ALIGN
+        order = range(len(self.free) - 1, -1, -1) if high else range(len(self.free))
+        for i in order:
+            o, ln = self.free[i]
+            if ln < n:
+                continue
+            off = o + ln - n if high else o
+            rest = (o, ln - n) if high else (o + n, ln - n)
+            if rest[1]:
+                self.free[i] = rest
+            else:
+                del self.free[i]
+            self.taken[off] = n
+            return off
+        return None
+
+    def release(self, off: int) -> None:
+        n = self.taken.pop(off)
+        i = 0
+        while i < len(self.free) and self.free[i][0] < off:
+            i += 1
+        self.free.insert(i, (off, n))
+        # merge with the right, then the left neighbour
+        if i + 1 < len(self.free) and off + n == self.free[i + 1][0]:
+            self.free[i] = (off, n + self.free[i + 1][1])
+            del self.free[i + 1]
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == off:
+            self.free[i - 1] = (self.free[i - 1][0], self.free[i - 1][1] + self.free[i][1])
+            del self.free[i]
+
+    def tensor(self, off: int, shape, dtype):
+        T = _lib.torch()
+        numel = int(np.prod(shape))
+        nb = numel * T.empty(0, dtype=dtype).element_size()
+        return self.buf[off: off + nb].view(dtype).view(tuple(shape))
+
+    def largest_free(self) -> int:
+        return max((ln for _, ln in self.free), default=0)
+
+
 class DeviceExecutor:
     """Runs the scheduler's launches on the GPU through the C ABI."""
 
@@ -304,6 +397,7 @@ class DeviceExecutor:
         self.capacity = dict(DEFAULT_CAPACITY if capacity is None else capacity)
         self.lanes: dict = {}
         self.results: dict = {}
+        self.arena = None
         nd = _lib.ctypes.c_int32()
         self._nd = {}
         for k in range(0, 9):
@@ -331,29 +425,56 @@ class DeviceExecutor:
         return (req.nshell * 12 * req.nside**2 * 4 + 2 * req.nshell * nl * (k + 1) * 8
                 + nl * self._nd[k] * 8)
 
+    # the arena
+    def bind(self, budget_bytes: float) -> None:
+        """One arena of the budget's size for every lane and request buffer."""
+        self.close()
+        self.arena = Arena(int(budget_bytes))
+
+    def close(self) -> None:
+        for key in list(self.lanes):
+            self.lanes[key].stream.synchronize()
+        self.lanes.clear()
+        self.arena = None
+        self.T.cuda.empty_cache()
+
+    def _place(self, parts, high: bool):
+        """Offsets in the arena for [(shape, dtype)], or MemoryError (nothing
+        kept) when they do not all fit now."""
+        T = self.T
+        offs = []
+        for shape, dtype in parts:
+            nb = int(np.prod(shape)) * T.empty(0, dtype=dtype).element_size()
+            off = self.arena.alloc(nb, high=high)
+            if off is None:
+                for o in offs:
+                    self.arena.release(o)
+                raise MemoryError(f"arena: no free range of {nb} bytes "
+                                  f"(largest {self.arena.largest_free()})")
+            offs.append(off)
+        return offs, [self.arena.tensor(o, shape, dtype) for o, (shape, dtype) in zip(offs, parts)]
+
     # lanes
     def open_lane(self, key) -> None:
         T = self.T
+        if self.arena is None:
+            raise RuntimeError("DeviceExecutor: bind(budget) before opening lanes")
         nside, lmax = key
         cap = self.lane_capacity(key)
         ln = _ReqState()
         ln.plan = _lib.get_plan(nside, lmax)
-        ln.stream = T.cuda.Stream()
         ln.cap = cap
-        with T.cuda.stream(ln.stream):
-            ln.alm = T.empty(_nalm(lmax) * cap, dtype=T.complex64, device="cuda")
-            ln.ws = T.empty(ln.plan.workspace_bytes(cap, 4), dtype=T.uint8, device="cuda")
-            ln.maps = T.empty((cap, 12 * nside * nside), dtype=T.float32, device="cuda")
+        ln.offs, (ln.alm, ln.ws, ln.maps) = self._place(
+            [((_nalm(lmax) * cap,), T.complex64), ((ln.plan.workspace_bytes(cap, 4),), T.uint8),
+             ((cap, 12 * nside * nside), T.float32)], high=True)
+        ln.stream = T.cuda.Stream()
         self.lanes[key] = ln
 
     def close_lane(self, key) -> None:
         ln = self.lanes.pop(key)
         ln.stream.synchronize()
-        del ln
-        # a lane's buffers are the largest blocks: hand them back to the
-        # device rather than keep them cached, so that lanes of other sizes
-        # opened later do not fail on a fragmented cache
-        self.T.cuda.empty_cache()
+        for o in ln.offs:
+            self.arena.release(o)
 
     # requests
     def open_request(self, req: SimRequest):
@@ -364,6 +485,9 @@ class DeviceExecutor:
         st = _ReqState()
         st.req = req
         st.key = self.lane_key(req)
+        # the output maps first: if they do not fit, nothing else is made
+        st.offs, (st.maps,) = self._place([((req.nshell, 12 * req.nside**2), T.float32)],
+                                          high=False)
         nl = req.lmax + 1
         gls = req.spectra()
         rows = np.stack([np.array(c) for c in _fields.cls2cov(gls, nl, req.nshell, req.ncorr)])
@@ -378,7 +502,6 @@ class DeviceExecutor:
             st.state = T.empty(nl * self._nd[req.ncorr], dtype=T.float64, device="cuda")
             st.flag = T.zeros(1, dtype=T.int32, device="cuda")
             st.flag_host = T.zeros(1, dtype=T.int32).pin_memory()
-            st.maps = T.empty((req.nshell, 12 * req.nside**2), dtype=T.float32, device="cuda")
         return st
 
     def close_request(self, st, keep: bool) -> None:
@@ -387,7 +510,9 @@ class DeviceExecutor:
         if int(st.flag_host[0]):
             raise ValueError(f"request {st.req.rid}: covariance matrix is not positive definite")
         if keep:
-            self.results[st.req.rid] = st.maps
+            self.results[st.req.rid] = st.maps.clone()  # out of the arena
+        for o in st.offs:
+            self.arena.release(o)
         st.maps = st.cov = st.w = st.state = st._host_cov = None
 
     # launches

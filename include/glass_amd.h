@@ -81,7 +81,11 @@ enum ga_tune_key {
                              (lmax + 1) 2 nshell floats; needs 64 ring rows of the
                              batch within 4 GiB); size the workspace under the same
                              setting; -1 = one chain; 0 = automatic (8 when
-                             lmax > 4096, else -3) */
+                             lmax > 4096, else -3) */,
+  GA_TUNE_M_GROUP = 8     /* split-FP16 synthesis block order: the m values of an XCD
+                             come in runs of this many consecutive m (1 or 4; 0 =
+                             automatic = 1), so that with 4 the four m of a 128-byte
+                             line of the ring buffer are written by one XCD's L2 */
 };
 int ga_set_tuning(int32_t key, int32_t value);
 int ga_get_tuning(int32_t key, int32_t *value);
@@ -136,8 +140,11 @@ int ga_alm2map_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
  * code (ga_noalloc_scratch_bytes: 128 MiB per device, held from the library's
  * load until the process ends), for callers that cannot allocate (code under
  * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).
- * Sizes whose workspace (ga_workspace_size(plan, nshell, 4)) exceeds the
- * scratch are rejected with GA_ERR_UNSUPPORTED before anything is launched;
+ * It runs the two-chunk accumulation in its compact form (the first chunk
+ * parked in the ring buffer, GA_TUNE_F16_FLUSH = -2) where the plan would
+ * use the form with a region of its own (-3): the maps are bit-identical.
+ * Sizes whose workspace under that form exceeds the scratch are rejected
+ * with GA_ERR_UNSUPPORTED before anything is launched;
  * the current device must be the plan's.  Every call on one GPU shares that
  * GPU's scratch, which keeps the last call's contents: calls that overlap on
  * one GPU (different streams, or host threads) race and must be serialised by

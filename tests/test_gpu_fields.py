@@ -190,10 +190,14 @@ def test_config1_cpu_case_parity():
                                     precision="fp32", batch=5))
     ref = list(O.generate([lambda x, v: O.lognormal(x, v)] * n, gls, nside, ncorr=1,
                           rng=np.random.default_rng(0)))
+    e64, e32 = [], []
     for g, g32, r in zip(got, got32, ref):
         rms = np.sqrt(np.mean(r**2))
-        assert np.max(np.abs(g - r)) / rms < 1e-12
-        assert np.max(np.abs(g32 - r)) / rms < 1e-5
+        e64.append(np.max(np.abs(g - r)) / rms)
+        e32.append(np.max(np.abs(g32 - r)) / rms)
+    print(f"C1 worst of {n} shells vs oracle: fp64 {max(e64):.2e}, fp32 {max(e32):.2e}",
+          flush=True)
+    assert max(e64) < 1e-12 and max(e32) < 1e-5
 
 
 def test_from_convergence_chain():

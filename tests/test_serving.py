@@ -215,3 +215,51 @@ def test_inflight_depth_is_bounded(depth):
             assert len(ln.inflight) <= depth
         clock.sleep(1e-3)
     assert len(sched.completed) == 3
+
+
+def test_executor_placement_failure_makes_the_request_wait():
+    """open_request / open_lane raising MemoryError (no room in the executor's
+    arena although the byte budget has room) delays the request until earlier
+    ones complete; every request is still served, in arrival order."""
+    clock = VClock()
+    ex = SimExecutor(clock, cap=4, lane_bytes=100, shell_bytes=10)
+    live = set()
+    orig_open, orig_close = ex.open_request, ex.close_request
+    refused = []
+
+    def open_request(req):
+        if len(live) >= 1:  # room for one request's buffers at a time
+            refused.append(req.rid)
+            raise MemoryError("no room")
+        live.add(req.rid)
+        return orig_open(req)
+
+    def close_request(st, keep):
+        live.discard(st["req"].rid)
+        orig_close(st, keep)
+
+    ex.open_request, ex.close_request = open_request, close_request
+    reqs = [SimRequest(i, 0.0, 16, 6, 32) for i in range(4)]
+    sched = _drive(reqs, ex, clock, budget=1e9)
+    assert [c.rid for c in sched.completed] == [0, 1, 2, 3]
+    assert refused and sched.rejected == []
+
+
+def test_arena_first_fit_and_merge():
+    """The device executor's arena bookkeeping (offsets only, no device)."""
+    from glass_amd.serving import Arena
+
+    a = Arena.__new__(Arena)
+    a.size, a.free, a.taken = 40960, [(0, 40960)], {}
+    lo = a.alloc(5000)
+    hi = a.alloc(4096, high=True)
+    assert lo == 0 and hi == 40960 - 4096 and a.taken[lo] == 8192
+    mid = a.alloc(8192)
+    assert mid == 8192
+    a.release(lo)
+    assert a.alloc(20000) == 16384  # first range large enough
+    a.release(mid)
+    a.release(16384)
+    a.release(hi)
+    assert a.free == [(0, 40960)] and not a.taken
+    assert a.alloc(50000) is None
