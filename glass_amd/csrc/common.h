@@ -53,7 +53,7 @@ struct ga_tuning {
   int ring_path = 0;    // FP32 ring synthesis: 0 auto (fused passes, register fold on wide rings), 1 radix-4 stages, 2 fused passes only
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
-  int m_group = 0;      // split-FP16 synthesis: consecutive m per XCD turn, 0 auto (= 1), 1, 4
+  int m_group = 0;      // split-FP16 synthesis: consecutive m per XCD turn, 0 auto (= 4), 1, 4
   int f16_flush = 0;    // split-FP16 synthesis accumulation: n > 0 master sums in registers
                         // flushed every n slabs (one column tile), -2 / -3 two l-chunks
                         // (the first parked in the ring buffer / in a region of its own,

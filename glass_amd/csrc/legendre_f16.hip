@@ -890,7 +890,7 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.nchunk = (flush == -2 || flush == -3) ? 2 : 1;
   g.nt = nt;
   g.blk_begin = 0;
-  g.mgroup = mgroup == 4 ? 4 : 1;
+  g.mgroup = mgroup == 1 ? 1 : 4;  // 0: automatic = 4
   // m slots per XCD: enough for m = 0 .. lmax (a multiple of 4 with runs of 4)
   const int slots = g.mgroup == 4 ? 4 * ((p->lmax + 1 + 31) / 32) : (p->lmax + 1 + 7) / 8;
   *nblk = int64_t(8) * slots * g.nrg * g.ncg;

@@ -41,14 +41,17 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_mfma_kernel(
     const double *__restrict__ z, const double *__restrict__ sth, const double2 *__restrict__ ab,
     const double *__restrict__ cm, const int32_t *__restrict__ ring_pnum,
     const int32_t *__restrict__ ring_pden, const float *__restrict__ alm, float *__restrict__ F) {
-  // XCD-aware decode: block b runs on XCD b % 8; XCD x takes m = x, x + 8, ...
-  // (largest l-range first), all (ring group, column group) tiles of one m
-  // back to back so they share that m's alm slice in the XCD's L2.
+  // XCD-aware decode: block b runs on XCD b % 8; XCD x takes the runs of four
+  // consecutive m starting at 4 x, 4 x + 32, ... (largest l-range first; the
+  // four m of a 128-byte line of the ring buffer go through one XCD's L2), all
+  // (ring group, column group) tiles of one m back to back so they share that
+  // m's alm slice in the XCD's L2.
   const int b = blockIdx.x;
   const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
-  const int m = 8 * (kq / G) + xcd;
+  const int mq = kq / G;
+  const int m = 32 * (mq >> 2) + 4 * xcd + (mq & 3);
   if (m > lmax) return;
-  const int rem = kq - (kq / G) * G;
+  const int rem = kq - mq * G;
   const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rp0 = rg * 256 + wave * 64;
@@ -255,7 +258,7 @@ int launch_legendre_synth_mfma(const ga_plan *p, const ga_tuning &tu, int S, con
   const int chunk = tu.mfma_chunk;
   const int nrg = (p->npairs + 255) / 256;
   const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
-  const int nm8 = (p->lmax + 1 + 7) / 8;
+  const int nm8 = 4 * ((p->lmax + 1 + 31) / 32);  // m slots per XCD, runs of 4
   const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
   GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
 #define GA_MFMA_LAUNCH(NT_, CH_, MW_)                                                         \

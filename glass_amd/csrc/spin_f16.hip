@@ -51,13 +51,16 @@ __global__ __launch_bounds__(256, MINW) void spin_synth_f16_kernel(
     const int32_t *__restrict__ ring_pnum, const int32_t *__restrict__ ring_pden,
     const int64_t *__restrict__ slab_off, const uint4 *__restrict__ B,
     const int32_t *__restrict__ colexp, float *__restrict__ F, int bpass) {
-  // XCD-aware decode as in the scalar stage: XCD x takes m = x, x + 8, ...,
-  // all (ring group, column group) tiles of one m back to back.
+  // XCD-aware decode as in the scalar stage: XCD x takes the runs of four
+  // consecutive m starting at 4 x, 4 x + 32, ... (the four m of a 128-byte
+  // line of the ring buffer are written through one XCD's L2), all (ring
+  // group, column group) tiles of one m back to back.
   const int b = blockIdx.x;
   const int xcd = b & 7, kq = b >> 3, G = nrg * ncg;
-  const int m = 8 * (kq / G) + xcd;
+  const int mq = kq / G;
+  const int m = 32 * (mq >> 2) + 4 * xcd + (mq & 3);
   if (m > lmax) return;
-  const int rem = kq - (kq / G) * G;
+  const int rem = kq - mq * G;
   const int rg = rem / ncg, cg = rem - (rem / ncg) * ncg;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int half = lane >> 5, col_in = lane & 31;
@@ -309,7 +312,7 @@ int launch_spin_synth_f16(const ga_plan *p, int spin, int S, const float *E, con
   const int nt = ncols > 32 ? 2 : 1;
   const int nrg = (p->npairs + 4 * kSpinPairs - 1) / (4 * kSpinPairs);
   const int ncg = (ncols + 32 * nt - 1) / (32 * nt);
-  const int nm8 = (p->lmax + 1 + 7) / 8;
+  const int nm8 = 4 * ((p->lmax + 1 + 31) / 32);  // m slots per XCD, runs of 4
   const int64_t nblk = int64_t(8) * nm8 * nrg * ncg;
   GA_REQUIRE(nblk < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
   for (int pass = 0; pass < (Bm ? 2 : 1); ++pass) {
