@@ -173,6 +173,7 @@ def extras_section(nside: int):
       writes int64 counts), the offset scan, the placement (reads counts,
       writes two float64 per point); points/s;
     * the pixel window of the geometry (spin 0 and 2, up to l = 4 nside);
+    * the pixel quadrature weights of the geometry (exact to l = 2 nside);
     * the Gaussian-spectra solver on the 64 lognormal spectra of the headline
       batch (host loop over device Gauss-Legendre transforms)."""
     import torch
@@ -241,6 +242,21 @@ def extras_section(nside: int):
     healpix.pixwin(nside, lmax=4 * nside, pol=True)
     out["pixwin"] = {"nside": nside, "lmax": 4 * nside, "pol": True,
                      "s": time.perf_counter() - t0}
+
+    hp_plan = _lib.get_plan(nside, 2 * nside)
+    n = _lib.ctypes.c_size_t()
+    _lib.call("ga_pixel_weights_workspace_size", hp_plan.h, _lib.ctypes.byref(n))
+    pws = _lib.workspace(n.value)
+    pw = torch.empty(npix, dtype=torch.float64, device="cuda")
+    it, res = _lib.ctypes.c_int32(), _lib.ctypes.c_double()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.call("ga_pixel_weights", hp_plan.h, 1e-12, 1000, _lib.ptr(pw), _lib.ctypes.byref(it),
+              _lib.ctypes.byref(res), _lib.ptr(pws), pws.numel(), st)
+    torch.cuda.synchronize()
+    out["pixel_weights"] = {"nside": nside, "exact_to_l": 2 * nside, "s": time.perf_counter() - t0,
+                            "cg_iterations": it.value, "relative_residual": res.value}
+    del pws, pw
 
     gls = shell_spectra(64, 2 * nside, amp=lambda i: 1.0 + 3.0 * i / 63, rho=(0.6, 0.3, 0.1),
                         scale=1500.0)

@@ -82,6 +82,11 @@ _SIGNATURES = {
     "ga_graph_destroy": [_vp],
     "ga_map2alm_f64": [_vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
     "ga_map2alm_batch_f32": [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
+    "ga_map2alm_weighted_f64": [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
+    "ga_map2alm_weighted_batch_f32": [_vp, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _sz, _vp],
+    "ga_pixel_weights_workspace_size": [_vp, ctypes.POINTER(_sz)],
+    "ga_pixel_weights": [_vp, ctypes.c_double, _i32, _vp, ctypes.POINTER(_i32),
+                         ctypes.POINTER(ctypes.c_double), _vp, _sz, _vp],
     "ga_alm2map_spin_f64": [_vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp],
     "ga_alm2map_spin_batch_f32": [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _sz, _vp],
     "ga_almxfl": [_i32, _i32, _i32, _vp, _vp, _i32, _vp, _vp],

@@ -60,13 +60,15 @@ def shear_factor(lmax: int, pixwin_ratio=None) -> np.ndarray:
 
 class ShearShells:
     """Batched convergence -> shear for S maps of one geometry, device resident:
-    FP32 analysis with Jacobi refinement (map2alm, iter passes), the
-    kappa -> gamma factor (almxfl), spin-2 synthesis with B = 0.  The batched
-    form of shear_from_convergence(kappa, lmax, discretized=False)
-    (glass/lensing.py:374-428); gamma[s] = (gamma1, gamma2) of map s."""
+    FP32 analysis with the pixel quadrature weights and Jacobi refinement
+    (map2alm(use_pixel_weights=True), iter passes; ``pixel_weights=False``:
+    uniform weights), the kappa -> gamma factor (almxfl), spin-2 synthesis
+    with B = 0.  The batched form of shear_from_convergence(kappa, lmax,
+    discretized=False) (glass/lensing.py:374-428); gamma[s] = (gamma1, gamma2)
+    of map s."""
 
     def __init__(self, nside: int, lmax: int, nshell: int, *, niter: int = 3,
-                 pixwin_ratio=None):
+                 pixwin_ratio=None, pixel_weights: bool = True):
         T = _lib.torch()
         _lib.require_gpu()
         self.nside, self.lmax, self.S, self.niter = int(nside), int(lmax), int(nshell), int(niter)
@@ -83,13 +85,19 @@ class ShearShells:
         self.tmp = T.empty((self.S, self.npix), dtype=T.float32, device="cuda")
         self.fl = T.as_tensor(shear_factor(self.lmax, pixwin_ratio)).to("cuda")
         self.gamma = T.empty((self.S, 2, self.npix), dtype=T.float32, device="cuda")
+        self.w = _hp.pixel_weights(self.nside) if pixel_weights else None
 
     def stage_analysis(self, kappa):
         if tuple(kappa.shape) != (self.S, self.npix) or not kappa.is_contiguous():
             raise ValueError("kappa must be a contiguous [nshell, npix] float32 device tensor")
-        _lib.call("ga_map2alm_batch_f32", self.plan.h, self.S, _lib.ptr(kappa),
-                  _lib.ptr(self.alm), self.niter, _lib.ptr(self.tmp), _lib.ptr(self.ws),
-                  self.ws.numel(), _lib.stream_ptr())
+        if self.w is not None:
+            _lib.call("ga_map2alm_weighted_batch_f32", self.plan.h, self.S, _lib.ptr(kappa),
+                      _lib.ptr(self.w), _lib.ptr(self.alm), self.niter, _lib.ptr(self.tmp),
+                      _lib.ptr(self.ws), self.ws.numel(), _lib.stream_ptr())
+        else:
+            _lib.call("ga_map2alm_batch_f32", self.plan.h, self.S, _lib.ptr(kappa),
+                      _lib.ptr(self.alm), self.niter, _lib.ptr(self.tmp), _lib.ptr(self.ws),
+                      self.ws.numel(), _lib.stream_ptr())
         _lib.call("ga_almxfl", self.lmax, self.S, 4, _lib.ptr(self.alm), _lib.ptr(self.fl),
                   int(self.fl.numel()), _lib.ptr(self.alm), _lib.stream_ptr())
 

@@ -15,6 +15,15 @@ __global__ void residual_kernel(int64_t n, const T *__restrict__ a, T *__restric
   if (i < n) b[i] = a[i] - b[i];
 }
 
+// dst = src * w * c per pixel (dst may be src): the per-pixel quadrature
+// weights of a weighted analysis, c undoing the analysis's uniform 4 pi / npix
+template <typename T>
+__global__ void weight_pixels_kernel(int64_t n, int64_t npix, const T *__restrict__ src,
+                                     const double *__restrict__ w, double c, T *dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = T(double(src[i]) * (w[i % npix] * c));
+}
+
 // same size as ga_workspace_size: fp32 ring buffers hold whole groups of
 // kMfmaGroup shells
 size_t ring_bytes(const ga_plan *p, int nbatch, int rb) {
@@ -204,12 +213,17 @@ extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float 
   return launch_ring_synth_f32(p, S, kMfmaGroup, F, maps, epi, (hipStream_t)stream);
 }
 
+// pixw: per-pixel quadrature weights (npix doubles, summing to 4 pi) in place
+// of the uniform 4 pi / npix, or NULL
 template <typename T>
 static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter, T *map_tmp,
-                        void *ws, size_t ws_bytes, hipStream_t s) {
+                        void *ws, size_t ws_bytes, hipStream_t s,
+                        const double *pixw = nullptr) {
   const int rb = int(sizeof(T));
   GA_REQUIRE(map && alm && ws && niter >= 0 && (niter == 0 || map_tmp), GA_ERR_INVALID,
              "invalid argument");
+  GA_REQUIRE(!pixw || map_tmp, GA_ERR_INVALID, "a weighted analysis needs map_tmp");
+  const double wc = double(p->npix) / (4.0 * M_PI);
   const size_t need = ring_bytes(p, S, rb) + partial_bytes(p, S, rb);
   GA_REQUIRE(ws_bytes >= need, GA_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
   T *F = (T *)ws;
@@ -246,6 +260,12 @@ static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter
       }
       src = map_tmp;
     }
+    if (pixw) {
+      hipLaunchKernelGGL(weight_pixels_kernel<T>, dim3(unsigned((npx + 255) / 256)), dim3(256), 0,
+                         s, npx, int64_t(p->npix), src, pixw, wc, map_tmp);
+      GA_LAUNCH_CHECK();
+      src = map_tmp;
+    }
     if (sizeof(T) == 8) {
       rc = launch_ring_anal_f64(p, S, (const double *)src, (double *)F, s);
       if (rc) return rc;
@@ -266,6 +286,25 @@ extern "C" int ga_map2alm_f64(const ga_plan *p, const double *map, double *alm, 
                               double *map_tmp, void *ws, size_t ws_bytes, void *stream) {
   GA_PLAN_OK(p);
   return map2alm_impl<double>(p, 1, map, alm, niter, map_tmp, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int ga_map2alm_weighted_f64(const ga_plan *p, const double *map,
+                                       const double *weights, double *alm, int32_t niter,
+                                       double *map_tmp, void *ws, size_t ws_bytes, void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(weights, GA_ERR_INVALID, "NULL weights");
+  return map2alm_impl<double>(p, 1, map, alm, niter, map_tmp, ws, ws_bytes, (hipStream_t)stream,
+                              weights);
+}
+
+extern "C" int ga_map2alm_weighted_batch_f32(const ga_plan *p, int32_t S, const float *maps,
+                                             const double *weights, float *alm, int32_t niter,
+                                             float *map_tmp, void *ws, size_t ws_bytes,
+                                             void *stream) {
+  GA_PLAN_OK(p);
+  GA_REQUIRE(S >= 1 && weights, GA_ERR_INVALID, "invalid argument");
+  return map2alm_impl<float>(p, S, maps, alm, niter, map_tmp, ws, ws_bytes, (hipStream_t)stream,
+                             weights);
 }
 
 extern "C" int ga_map2alm_batch_f32(const ga_plan *p, int32_t S, const float *maps, float *alm,

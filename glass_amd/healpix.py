@@ -156,21 +156,52 @@ def alm2map(alms, nside: int, *, inplace: bool = False, lmax: int | None = None,
     return outs[0] if single else outs
 
 
+_pixel_weight_cache: dict = {}
+
+
+def pixel_weights(nside: int, *, tol: float = 1e-12, maxiter: int = 1000):
+    """Per-pixel quadrature weights of ``nside`` (float64 device tensor of
+    npix), computed on the GPU (``ga_pixel_weights``) in place of healpy's
+    weight files: the smallest correction of the uniform weights 4 pi / npix
+    for which the pixel sum of Y*_lm equals its integral for every
+    l <= 2 nside, so the weighted analysis of a map band-limited to nside is
+    exact.  Computed once per (nside, device)."""
+    T = _lib.torch()
+    _lib.require_gpu()
+    nside = int(nside)
+    key = (nside, T.cuda.current_device())
+    if key not in _pixel_weight_cache:
+        plan = _lib.get_plan(nside, 2 * nside)
+        n = _lib.ctypes.c_size_t()
+        _lib.call("ga_pixel_weights_workspace_size", plan.h, _lib.ctypes.byref(n))
+        ws = _lib.workspace(n.value)
+        w = T.empty(nside2npix(nside), dtype=T.float64, device="cuda")
+        it = _lib.ctypes.c_int32()
+        res = _lib.ctypes.c_double()
+        _lib.call("ga_pixel_weights", plan.h, float(tol), int(maxiter), _lib.ptr(w),
+                  _lib.ctypes.byref(it), _lib.ctypes.byref(res), _lib.ptr(ws), ws.numel(),
+                  _lib.stream_ptr())
+        if res.value > 100 * tol:
+            warnings.warn(f"pixel weights for nside {nside}: relative residual {res.value:.2e} "
+                          f"after {it.value} iterations", stacklevel=2)
+        _pixel_weight_cache[key] = w
+    return _pixel_weight_cache[key]
+
+
 def map2alm(maps, *, lmax: int | None = None, pol: bool = True,
             use_pixel_weights: bool = False, iter: int = 3,  # noqa: A002
             precision: str = "fp64"):
     """glass/healpix.py:232-276 (healpy.map2alm; RING input, mmax = lmax,
     default lmax = 3 nside - 1, ``iter`` Jacobi refinement passes).
 
-    Weights: uniform 4 pi / npix per pixel.  healpy's ``use_pixel_weights``
-    tables are data files that are not available here; the datapath is checked
-    exactly like the reference, and the weighting falls back to uniform
-    weights refined by the Jacobi iterations.
+    Weights: uniform 4 pi / npix per pixel, or with ``use_pixel_weights`` the
+    per-pixel quadrature weights of ``pixel_weights`` (computed on the GPU,
+    where healpy reads them from data files; the datapath is still checked
+    exactly like the reference).  The weighted analysis runs in FP64 (also for
+    ``precision="fp32"``, whose result is then rounded to complex64).
     """
     if use_pixel_weights:
         _get_healpy_datapath()
-        warnings.warn("use_pixel_weights: pixel-weight tables are not available; using uniform "
-                      "weights with Jacobi refinement", stacklevel=2)
     T = _lib.torch()
     single = getattr(maps, "ndim", None) == 1
     if not single and pol and len(_as_list(maps)) == 3:
@@ -181,17 +212,27 @@ def map2alm(maps, *, lmax: int | None = None, pol: bool = True,
         lmax = 3 * nside - 1
     plan = _lib.get_plan(nside, lmax)
     outs = []
-    if precision == "fp64":
+    if precision not in ("fp64", "fp32"):
+        raise ValueError(f"unknown precision {precision!r}")
+    if precision == "fp64" or use_pixel_weights:
         ws = _lib.workspace(plan.workspace_bytes(1, 8, analysis=True))
+        w = pixel_weights(nside) if use_pixel_weights else None
         for mp in lst:
             d, kind = _lib.to_device(mp, T.float64)
             alm = T.empty(nalm(lmax), dtype=T.complex128, device="cuda")
-            tmp = T.empty_like(d) if iter > 0 else None
-            _lib.call("ga_map2alm_f64", plan.h, _lib.ptr(d), _lib.ptr(alm), int(iter),
-                      _lib.ptr(tmp) if tmp is not None else None, _lib.ptr(ws), ws.numel(),
-                      _lib.stream_ptr())
+            tmp = T.empty_like(d) if (iter > 0 or w is not None) else None
+            if w is not None:
+                _lib.call("ga_map2alm_weighted_f64", plan.h, _lib.ptr(d), _lib.ptr(w),
+                          _lib.ptr(alm), int(iter), _lib.ptr(tmp), _lib.ptr(ws), ws.numel(),
+                          _lib.stream_ptr())
+            else:
+                _lib.call("ga_map2alm_f64", plan.h, _lib.ptr(d), _lib.ptr(alm), int(iter),
+                          _lib.ptr(tmp) if tmp is not None else None, _lib.ptr(ws), ws.numel(),
+                          _lib.stream_ptr())
+            if precision == "fp32":
+                alm = alm.to(T.complex64)
             outs.append(_lib.from_device(alm, kind))
-    elif precision == "fp32":
+    else:
         devs = []
         kind = "numpy"
         for mp in lst:
@@ -200,8 +241,6 @@ def map2alm(maps, *, lmax: int | None = None, pol: bool = True,
         batch = T.stack(devs, dim=0).contiguous()
         alm = map2alm_batch_f32(batch, lmax, iter)
         outs = [_lib.from_device(alm[:, i].contiguous(), kind) for i in range(len(lst))]
-    else:
-        raise ValueError(f"unknown precision {precision!r}")
     return outs[0] if single else outs
 
 

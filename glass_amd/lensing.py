@@ -49,7 +49,7 @@ def from_convergence(kappa, lmax: int | None = None, *, potential: bool = False,
     nside = hp.get_nside(kappa)
     if lmax is None:
         lmax = 3 * nside - 1
-    alm = hp.map2alm(kappa, lmax=lmax, pol=False, use_pixel_weights=False, precision=precision)
+    alm = hp.map2alm(kappa, lmax=lmax, pol=False, use_pixel_weights=True, precision=precision)
     ell = _ell(lmax)
     results = ()
     fl = np.zeros(lmax + 1)
@@ -83,7 +83,7 @@ def shear_from_convergence(kappa, lmax: int | None = None, *, discretized: bool 
     nside = hp.get_nside(kappa)
     if lmax is None:
         lmax = 3 * nside - 1
-    alm = hp.map2alm(kappa, lmax=lmax, pol=False, use_pixel_weights=False, precision=precision)
+    alm = hp.map2alm(kappa, lmax=lmax, pol=False, use_pixel_weights=True, precision=precision)
     ell = _ell(lmax)
     fl = np.sqrt((ell + 2) * (ell + 1) * ell * np.clip(ell - 1, 0, None))
     fl /= np.clip(ell * (ell + 1), 1, None)

@@ -225,6 +225,40 @@ int ga_graph_destroy(ga_graph *graph);
 int ga_map2alm_f64(const ga_plan *plan, const double *map, double *alm, int32_t niter,
                    double *map_tmp, void *workspace, size_t ws_bytes, void *stream);
 
+/* FP64 scalar analysis with per-pixel quadrature weights (npix doubles that
+ * sum to 4 pi, e.g. from ga_pixel_weights) in place of the uniform
+ * 4 pi / npix, and healpy's Jacobi refinement with the same weights
+ * (alm += A_w(map - S(alm)), niter times).  map_tmp (npix doubles) is
+ * required.  glass/healpix.py:232-276 map2alm(use_pixel_weights=True), as
+ * called at glass/lensing.py:306 and :408. */
+int ga_map2alm_weighted_f64(const ga_plan *plan, const double *map, const double *weights,
+                            double *alm, int32_t niter, double *map_tmp, void *workspace,
+                            size_t ws_bytes, void *stream);
+
+/* Pixel quadrature weights of the plan's nside, computed instead of read from
+ * healpy's weight files (which map2alm(use_pixel_weights=True) of
+ * glass/healpix.py:232-276 needs): the smallest correction of the uniform
+ * weights 4 pi / npix for which sum_p w_p Y*_lm(p) = sqrt(4 pi) delta_l0
+ * delta_m0 for every l <= the plan's lmax (which must be <= 2 nside), by
+ * conjugate gradients on the plan's FP64 synthesis and analysis until the
+ * residual falls below tol (relative) or maxiter iterations.  Writes npix
+ * weights, the iteration count and the final relative residual (either may
+ * be NULL).  Synchronises the stream (it reads inner products back each
+ * iteration): a set-up call, not capturable. */
+int ga_pixel_weights_workspace_size(const ga_plan *plan, size_t *bytes);
+int ga_pixel_weights(const ga_plan *plan, double tol, int32_t maxiter, double *weights,
+                     int32_t *iterations, double *residual, void *workspace, size_t ws_bytes,
+                     void *stream);
+
+/* ga_map2alm_batch_f32 with per-pixel quadrature weights (npix doubles, the
+ * same for every map) in place of the uniform 4 pi / npix, also in the
+ * Jacobi passes; map_tmp is required.  The batched form of
+ * map2alm(use_pixel_weights=True) at glass/lensing.py:306 and :408. */
+int ga_map2alm_weighted_batch_f32(const ga_plan *plan, int32_t nshell, const float *maps,
+                                  const double *weights, float *alm, int32_t niter,
+                                  float *map_tmp, void *workspace, size_t ws_bytes,
+                                  void *stream);
+
 /* FP32 batched analysis of S maps [S][npix] into [idx][S] alm (niter Jacobi
  * passes; the residual synthesis uses the FP32-MFMA path).
  * map_tmp: S npix float32 scratch. */

@@ -56,6 +56,14 @@ Pinning (what checks each function, all in ``tests/test_oracle.py``):
                     symmetry-reduced pixel set) is checked against it, against
                     itself over all pixels, and against w_0 = 1.
 
+* pixel_weights  -- dense minimum-norm least squares on the harmonic matrix of
+                    the pixel centres (scipy sph_harm_y); pinned by the
+                    quadrature conditions it solves (sum_p w_p Y*_lm =
+                    sqrt(4 pi) delta), by the correction lying in the row
+                    space of the conditions (minimum norm), by Gauss-Legendre-
+                    like exactness for band-limited products, and by the
+                    HEALPix symmetries of the result.
+
 Parity unpinned: healpy's pixel-weight files (use_pixel_weights=True) and the
 tabulated pixel windows that healpy ships (pixwin here is computed from its
 definition; the tables themselves are not available to compare with).
@@ -940,3 +948,89 @@ def gl_corrtocl(corr) -> np.ndarray:
     n = corr.size
     x, w = gl_nodes(n)
     return 2 * np.pi * (_legendre_table(x, n) @ (w * corr))
+
+
+# --------------------------------------------------------------------------
+# Pixel quadrature weights (computed in place of healpy's weight files that
+# map2alm(use_pixel_weights=True) reads, glass/healpix.py:232-276): the
+# smallest correction of the uniform weights 4 pi / npix for which the pixel
+# sum of every Y*_lm with l <= lmax is the integral sqrt(4 pi) delta_l0.
+
+
+def ylm_matrix(nside: int, lmax: int) -> np.ndarray:
+    """Y_lm (Condon-Shortley) at every pixel centre, [npix, nalm] in healpy order."""
+    from scipy.special import sph_harm_y
+
+    th, ph = pixel_angles(nside)
+    Y = np.empty((th.size, nalm(lmax)), dtype=np.complex128)
+    i = 0
+    for m in range(lmax + 1):
+        for l in range(m, lmax + 1):
+            Y[:, i] = sph_harm_y(l, m, th, ph)
+            i += 1
+    return Y
+
+
+def _constraint_rows(Y: np.ndarray, lmax: int) -> np.ndarray:
+    """Real rows of the conditions: Y_l0 (real), then Re and Im of Y_lm, m > 0."""
+    return np.concatenate([Y[:, : lmax + 1].real.T, Y[:, lmax + 1:].real.T,
+                           Y[:, lmax + 1:].imag.T])
+
+
+def _alm_dot(y: np.ndarray, z: np.ndarray, lmax: int) -> float:
+    f = np.full(y.size, 2.0)
+    f[: lmax + 1] = 1.0
+    return float(np.sum(f * (y.real * z.real + y.imag * z.imag)))
+
+
+def pixel_weights_cg(nside: int, lmax: int, tol: float = 1e-13, maxiter: int = 2000):
+    """The same weights by conjugate gradients on C S lambda = b - C(w0) with
+    this module's C transforms (C = (npix / 4 pi) map2alm without iterations,
+    S = alm2map), for sizes where the dense solve is too large."""
+    npix = nside2npix(nside)
+    kappa = npix / (4 * np.pi)
+
+    def C(f):
+        c = kappa * map2alm(f, lmax, niter=0)
+        c[: lmax + 1] = c[: lmax + 1].real
+        return c
+
+    b = np.zeros(nalm(lmax), dtype=np.complex128)
+    b[0] = np.sqrt(4 * np.pi)
+    r = b - C(np.full(npix, 4 * np.pi / npix))
+    lam = np.zeros_like(r)
+    p = r.copy()
+    rr = rr0 = _alm_dot(r, r, lmax)
+    for _ in range(maxiter):
+        if rr <= tol * tol * rr0:
+            break
+        q = C(alm2map(p, nside, lmax))
+        alpha = rr / _alm_dot(p, q, lmax)
+        lam += alpha * p
+        r -= alpha * q
+        rn = _alm_dot(r, r, lmax)
+        p = r + (rn / rr) * p
+        rr = rn
+    return 4 * np.pi / npix + alm2map(lam, nside, lmax)
+
+
+def pixel_weights(nside: int, lmax: int) -> np.ndarray:
+    Y = ylm_matrix(nside, lmax)
+    A = _constraint_rows(Y, lmax)
+    b = np.zeros(A.shape[0])
+    b[0] = np.sqrt(4 * np.pi)
+    w0 = np.full(A.shape[1], 4 * np.pi / A.shape[1])
+    dw = np.linalg.lstsq(A, b - A @ w0, rcond=None)[0]
+    return w0 + dw
+
+
+def map2alm_weighted(m: np.ndarray, w: np.ndarray, lmax: int, niter: int = 3) -> np.ndarray:
+    """a_lm = sum_p w_p f_p Y*_lm(p), then healpy's Jacobi passes
+    a += A_w(f - S a); A_w(f) is the uniform-weight analysis of
+    f w npix / (4 pi)."""
+    nside = npix2nside(m.size)
+    c = w * (m.size / (4 * np.pi))
+    a = map2alm(m * c, lmax, niter=0)
+    for _ in range(niter):
+        a = a + map2alm((m - alm2map(a, nside, lmax)) * c, lmax, niter=0)
+    return a

@@ -65,8 +65,9 @@ def test_shear_batch_matches_oracle(nside, lmax, S):
     eng = ShearShells(nside, lmax, S)
     gamma = eng.run(torch.as_tensor(np.stack(kappas)).cuda()).cpu().numpy()
     fl = O.kappa_to_shear_factor(lmax)
+    w = O.pixel_weights_cg(nside, 2 * nside)
     for s in range(S):
-        a = O.almxfl(O.map2alm(kappas[s].astype(np.float64), lmax, niter=3), fl)
+        a = O.almxfl(O.map2alm_weighted(kappas[s].astype(np.float64), w, lmax, niter=3), fl)
         q, u = O.alm2map_spin([a, np.zeros_like(a)], nside, 2, lmax)
         assert relerr(gamma[s, 0], q) < 1e-5
         assert relerr(gamma[s, 1], u) < 1e-5
@@ -100,7 +101,7 @@ def _check_shells(nside, lmax, S, shells):
         e_map, r_map = _trelerr(eng.maps[s], ref)
         # FP64 shear of the FP32 map (same input, so only the shear's own error counts)
         kappa = eng.maps[s].double()
-        alm = hp.map2alm(kappa, lmax=lmax, pol=False, precision="fp64")
+        alm = hp.map2alm(kappa, lmax=lmax, pol=False, precision="fp64", use_pixel_weights=True)
         alm = hp.almxfl(alm, fl)
         q, u = hp.alm2map_spin([alm, torch.zeros_like(alm)], nside, 2, lmax)
         e_q, _ = _trelerr(eng.gamma[s, 0], q)

@@ -209,8 +209,9 @@ def test_from_convergence_chain():
     res = glass_amd.from_convergence(kappa, lmax, potential=True, deflection=True, shear=True,
                                      discretized=False)
     assert len(res) == 3
-    # oracle chain (glass/lensing.py:306-367) with healpy's iter=3 analysis
-    a = O.map2alm(kappa, lmax, niter=3)
+    # oracle chain (glass/lensing.py:306-367): the pixel-weighted analysis with
+    # healpy's iter=3 passes
+    a = O.map2alm_weighted(kappa, O.pixel_weights_cg(nside, 2 * nside), lmax, niter=3)
     ell = np.arange(lmax + 1, dtype=float)
     fl = np.zeros(lmax + 1)
     fl[1:] = -2 / (ell[1:] * (ell[1:] + 1))

@@ -110,7 +110,7 @@ def test_from_convergence_reference_defaults():
     (gamma,) = glass_amd.from_convergence(kappa, shear=True)
     lmax = 3 * n_side - 1
     ratio = O.pixwin_pairs(n_side, lmax, 2) / O.pixwin_pairs(n_side, lmax, 0)
-    a = O.map2alm(kappa, lmax, niter=3)
+    a = O.map2alm_weighted(kappa, O.pixel_weights_cg(n_side, 2 * n_side), lmax, niter=3)
     a = O.almxfl(a, O.kappa_to_shear_factor(lmax, ratio))
     q, u = O.alm2map_spin([a, np.zeros_like(a)], n_side, 2, lmax)
     ref = q + 1j * u

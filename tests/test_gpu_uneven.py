@@ -127,7 +127,8 @@ def test_c4_geometries_graph_eager_and_fp64():
         fl[2:] = -np.sqrt((ell[2:] - 1) * (ell[2:] + 2) / (ell[2:] * (ell[2:] + 1)))
         for j in range(eng.S):
             e_map = rel(eng.maps[j], eng.fp64_reference_map(j))
-            alm = hp.map2alm(eng.maps[j].double(), lmax=lmax, pol=False, precision="fp64")
+            alm = hp.map2alm(eng.maps[j].double(), lmax=lmax, pol=False, precision="fp64",
+                             use_pixel_weights=True)
             alm = hp.almxfl(alm, fl)
             q, u = hp.alm2map_spin([alm, torch.zeros_like(alm)], nside, 2, lmax)
             e_q, e_u = rel(eng.gamma[j, 0], q), rel(eng.gamma[j, 1], u)

@@ -755,3 +755,62 @@ def test_spectra_order_helpers():
     assert list(enumerate_spectra([[1, 2, 3], [4, 5, 6], [7, 8, 9]])) == [
         (0, 0, [1, 2, 3]), (1, 1, [4, 5, 6]), (1, 0, [7, 8, 9])]
     assert spectra_indices(3).tolist() == [[0, 0], [1, 1], [1, 0], [2, 2], [2, 1], [2, 0]]
+
+
+# ---- pixel quadrature weights ------------------------------------------------
+
+
+@pytest.mark.parametrize("nside", [1, 2, 4])
+def test_pixel_weights_oracle(nside):
+    """The dense solve: the quadrature conditions hold, the correction is the
+    minimum-norm one (it lies in the span of the conditions' rows), the weights
+    sum to 4 pi and carry the HEALPix symmetries (rotation by pi/2 within a
+    ring, phi -> -phi, the equatorial mirror), and the weighted analysis of a
+    map band-limited to nside is exact without iterations."""
+    L = 2 * nside
+    w = O.pixel_weights(nside, L)
+    Y = O.ylm_matrix(nside, L)
+    A = O._constraint_rows(Y, L)
+    b = np.zeros(A.shape[0])
+    b[0] = np.sqrt(4 * np.pi)
+    assert np.max(np.abs(A @ w - b)) < 1e-12
+    assert abs(w.sum() - 4 * np.pi) < 1e-12
+    dw = w - 4 * np.pi / w.size
+    coef = np.linalg.lstsq(A.T, dw, rcond=None)[0]
+    assert np.max(np.abs(A.T @ coef - dw)) < 1e-12
+    # symmetries, ring by ring
+    for r in range(1, 4 * nside):
+        start, n, z, _, phi0 = O.ring_info(nside, r)
+        wr = w[start:start + n]
+        assert np.allclose(wr, np.roll(wr, n // 4), rtol=0, atol=1e-13)
+        # phi -> -phi: pixel j -> n - 1 - j (phi0 = pi / n) or -j (phi0 = 0)
+        mirror = wr[::-1] if phi0 > 0 else np.roll(wr[::-1], 1)
+        assert np.allclose(wr, mirror, rtol=0, atol=1e-13)
+        s2, n2, _, _, _ = O.ring_info(nside, 4 * nside - r)
+        assert np.allclose(np.sort(wr), np.sort(w[s2:s2 + n2]), rtol=0, atol=1e-13)
+    # exact analysis of a band-limited map (lmax = nside) with no iteration
+    rng = np.random.default_rng(nside)
+    lm = nside
+    alm = rng.standard_normal(O.nalm(lm)) + 1j * rng.standard_normal(O.nalm(lm))
+    alm[: lm + 1] = alm[: lm + 1].real
+    m = O.alm2map(alm, nside, lm)
+    got = O.map2alm_weighted(m, w, lm, niter=0)
+    assert np.max(np.abs(got - alm)) < 1e-11
+
+
+@pytest.mark.parametrize("nside", [2, 4, 8])
+def test_pixel_weights_cg_equals_dense(nside):
+    """The oracle's conjugate-gradient form (for larger nside) against the
+    dense minimum-norm solve, and the weighted analysis through the uniform
+    one against the dense harmonic matrix."""
+    L = 2 * nside
+    wd = O.pixel_weights(nside, L)
+    wc = O.pixel_weights_cg(nside, L)
+    assert np.max(np.abs(wc - wd)) < 1e-12 * np.max(np.abs(wd)) * 10
+    rng = np.random.default_rng(nside)
+    m = rng.standard_normal(wd.size)
+    lm = 3 * nside - 1
+    Y = O.ylm_matrix(nside, lm)
+    ref = Y.conj().T @ (wd * m)
+    got = O.map2alm_weighted(m, wd, lm, niter=0)
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref)) * 10
