@@ -36,6 +36,7 @@ TUNE_SPIN_PATH = 5
 TUNE_ANAL_PATH = 6
 TUNE_F16_FLUSH = 7
 TUNE_M_GROUP = 8
+TUNE_CHUNK_MIN = 9
 
 T_NORMAL = 0
 T_LOGNORMAL = 1

@@ -54,6 +54,8 @@ struct ga_tuning {
   int spin_path = 0;    // FP32 spin synthesis: 0 auto (= split FP16 MFMA), 1 VALU, 2 split FP16 MFMA
   int anal_path = 0;    // FP32 analysis Legendre stage: 0 auto (= split FP16 MFMA), 1 VALU, 2 MFMA
   int m_group = 0;      // split-FP16 synthesis: consecutive m per XCD turn, 0 auto (= 4), 1, 4
+  int chunk_min = 0;    // two-chunk form: split only l-ranges of at least this many slabs
+                        // (0 auto = kTwoChunkMinSlabs)
   int f16_flush = 0;    // split-FP16 synthesis accumulation: n > 0 master sums in registers
                         // flushed every n slabs (one column tile), -2 / -3 two l-chunks
                         // (the first parked in the ring buffer / in a region of its own,
@@ -66,6 +68,13 @@ struct ga_tuning {
 constexpr int kAutoFlushLmax = 4096;
 constexpr int kAutoFlushSlabs = 8;
 constexpr int kAutoChunks = -3;
+// The two-chunk form splits the l-range of a wave only when at least this
+// many slabs (of 32 rows) remain after the dead ones: the long chains of the
+// small m set the FP32 rounding error of a map, shorter ones add little to
+// it.  Measured at nside 2048, 64 shells: split from 8 slabs 33.3 ms, worst
+// shell 6.8e-6; from 32: 33.0 ms, 6.7e-6; from 64: 32.4 ms, 7.2e-6; from 96:
+// 31.9 ms, 7.2e-6 (median 5.8e-6 against 5.3e-6).
+constexpr int kTwoChunkMinSlabs = 64;
 ga_tuning &tuning();  // the process default
 
 // l rows per slab of the FP16-split MFMA Legendre stage (K = 16 per parity)

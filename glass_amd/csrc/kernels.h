@@ -28,10 +28,11 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 // sums every n slabs, -2 two l-chunks with the first parked in F and the O
 // half of its equator row at park ((lmax + 1) 2 S floats), -3 two l-chunks
 // with the first parked at park, one 16 nt KiB slot per wave of the grid)
-// mgroup: consecutive m per XCD turn (1 or 4)
+// mgroup: consecutive m per XCD turn (1 or 4); chunk_min: least l-range in
+// slabs that the two-chunk form splits (0: kTwoChunkMinSlabs)
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s, int nt, int flush,
-                                 void *park, int mgroup);
+                                 void *park, int mgroup, int chunk_min);
 // the batched Legendre stage on the path chosen by the tuning set tu (a
 // snapshot of the plan's, the same for the preparation and the contraction
 // of one synthesis); ws is the FP32 workspace with F at its start

@@ -50,6 +50,7 @@ using f16::split_pair;
 
 constexpr int kColTop = 13;  // column maxima are scaled below 2^13
 
+
 __global__ void fill_u32_kernel(uint32_t *__restrict__ p, int n, uint32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -420,7 +421,8 @@ struct f16_geom {
   float4 *part;   // two chunks parked in a region of their own (sep)
   int sep;
   int lmax, npairs, nrings, ncols, nct, nrg, ncg, lam_exp, flush, nchunk, nt, blk_begin;
-  int mgroup;  // consecutive m per XCD turn (1 or 4)
+  int mgroup;     // consecutive m per XCD turn (1 or 4)
+  int chunk_min;  // two-chunk form: least l-range (slabs) that is split
 };
 
 struct f16_group {
@@ -627,13 +629,13 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
     }
   };
 
-  // Two-chunk form (nchunk == 2): the l-range left after the dead slabs is
-  // cut in two halves; the first half's accumulators are parked in F
+  // Two-chunk form (nchunk == 2): the l-range left after the dead slabs, if
+  // at least g.chunk_min slabs long, is cut in two halves; the first half's accumulators are parked in F
   // (park_tile), the second half starts from zero, and the parked sums are
   // added back before the epilogue: two shorter FP32 rounding chains.  The
   // two halves are two copies of the slab loop, not one loop with a cut in
   // it, so the parking code sits outside every loop.
-  const bool two = !FLUSH && nchunk == 2 && nsl - s >= 8;
+  const bool two = !FLUSH && nchunk == 2 && nsl - s >= g.chunk_min;
   run_slabs(two ? s + (nsl - s) / 2 : nsl);
   if (two) {
     if (SEP) park_lanes<NT, false>(acc, g.part + (int64_t(b) * 4 + wave) * (16 * NT * 64) + lane);
@@ -863,7 +865,7 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 // block count
 static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
                              const int32_t *colexp, int nt, int flush, void *park,
-                             int mgroup, int64_t *nblk) {
+                             int mgroup, int chunk_min, int64_t *nblk) {
   f16_geom g{};
   g.z = p->d_z;
   g.sth = p->d_sth;
@@ -891,6 +893,7 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.nt = nt;
   g.blk_begin = 0;
   g.mgroup = mgroup == 1 ? 1 : 4;  // 0: automatic = 4
+  g.chunk_min = chunk_min > 0 ? chunk_min : kTwoChunkMinSlabs;
   // m slots per XCD: enough for m = 0 .. lmax (a multiple of 4 with runs of 4)
   const int slots = g.mgroup == 4 ? 4 * ((p->lmax + 1 + 31) / 32) : (p->lmax + 1 + 7) / 8;
   *nblk = int64_t(8) * slots * g.nrg * g.ncg;
@@ -899,7 +902,7 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t stream, int nt, int flush,
-                                 void *park, int mgroup) {
+                                 void *park, int mgroup, int chunk_min) {
   const int ncols = 2 * S;
   const int nct = (ncols + 31) / 32;
   const uint4 *B = reinterpret_cast<const uint4 *>(bws);
@@ -929,7 +932,8 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
     return GA_OK;
   }
   int64_t nblk2 = 0;
-  const f16_geom g = f16_geometry(p, S, F, bws, colexp, nt, flush, park, mgroup, &nblk2);
+  const f16_geom g =
+      f16_geometry(p, S, F, bws, colexp, nt, flush, park, mgroup, chunk_min, &nblk2);
   GA_REQUIRE(nblk2 < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
 #define GA_F16_LAUNCH(NT_, MW_, FL_, SEP_)                                                      \
   hipLaunchKernelGGL((legendre_synth_f16_kernel<NT_, MW_, FL_, SEP_>), dim3(unsigned(nblk2)),  \
@@ -970,7 +974,7 @@ int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, co
     int64_t nblk = 0;
     grp.g[i] = f16_geometry(p, S[i], reinterpret_cast<float *>(base), base + w.b_off,
                             reinterpret_cast<const int32_t *>(base + w.e_off), nt, acc_form,
-                            base + w.p_off, tu.m_group, &nblk);
+                            base + w.p_off, tu.m_group, tu.chunk_min, &nblk);
     grp.g[i].blk_begin = int(begin);  // a multiple of 8: XCD mapping as in one launch
     begin += nblk;
     GA_REQUIRE(begin < (int64_t(1) << 31), GA_ERR_UNSUPPORTED, "grid too large");
@@ -1013,7 +1017,7 @@ int launch_legendre_contract_batch(const ga_plan *p, const ga_tuning &tu, int S,
   const int acc_form = f16_accumulation(p, tu, S);
   return launch_legendre_contract_f16(
       p, S, F, base + w.b_off, reinterpret_cast<const int32_t *>(base + w.e_off), stream,
-      tu.mfma_nt, acc_form, const_cast<char *>(base) + w.p_off, tu.m_group);
+      tu.mfma_nt, acc_form, const_cast<char *>(base) + w.p_off, tu.m_group, tu.chunk_min);
 }
 
 int launch_legendre_synth_batch(const ga_plan *p, int S, const float *alm, void *ws,

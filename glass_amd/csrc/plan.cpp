@@ -213,6 +213,10 @@ int tuning_set(ga_tuning &t, int32_t key, int32_t value) {
       GA_REQUIRE(value >= 0 && value <= 2, GA_ERR_INVALID, "analysis path must be 0, 1 or 2");
       t.anal_path = value;
       return GA_OK;
+    case GA_TUNE_CHUNK_MIN:
+      GA_REQUIRE(value >= 0 && value <= 4096, GA_ERR_INVALID, "chunk min out of range");
+      t.chunk_min = value;
+      return GA_OK;
     case GA_TUNE_M_GROUP:
       GA_REQUIRE(value == 0 || value == 1 || value == 4, GA_ERR_INVALID, "m group must be 0, 1 or 4");
       t.m_group = value;
@@ -237,6 +241,7 @@ int tuning_get(const ga_tuning &t, int32_t key, int32_t *value) {
     case GA_TUNE_ANAL_PATH: *value = t.anal_path; return GA_OK;
     case GA_TUNE_F16_FLUSH: *value = t.f16_flush; return GA_OK;
     case GA_TUNE_M_GROUP: *value = t.m_group; return GA_OK;
+    case GA_TUNE_CHUNK_MIN: *value = t.chunk_min; return GA_OK;
     default: GA_REQUIRE(false, GA_ERR_INVALID, "unknown tuning key %d", key);
   }
 }

@@ -28,6 +28,7 @@ def _engine(nside, lmax, S, seed, nt, flush):
     eng.plan.set_tuning(_lib.TUNE_MFMA_PATH, 2)
     eng.plan.set_tuning(_lib.TUNE_MFMA_TILES, nt)
     eng.plan.set_tuning(_lib.TUNE_F16_FLUSH, flush)
+    eng.plan.set_tuning(_lib.TUNE_CHUNK_MIN, 2)  # park even at these small lmax
     # the workspace is sized for the plan's accumulation form
     eng.F = _lib.workspace(eng.plan.workspace_bytes(S, 4))
     eng.stage_fields()

@@ -104,12 +104,16 @@ def test_fp32_accumulation_forms_are_consistent(flush):
     old = plan.get_tuning(_lib.TUNE_F16_FLUSH)
     out = {}
     try:
+        # split every l-range of at least two slabs (the default leaves the
+        # 33 slabs of lmax 1024 in one chain)
+        plan.set_tuning(_lib.TUNE_CHUNK_MIN, 2)
         for fl in (-1, flush):
             plan.set_tuning(_lib.TUNE_F16_FLUSH, fl)
             out[fl] = np.asarray(hp.alm2map(alms, nside, pol=False, precision="fp32"),
                                  dtype=np.float64)
     finally:
         plan.set_tuning(_lib.TUNE_F16_FLUSH, old)
+        plan.set_tuning(_lib.TUNE_CHUNK_MIN, 0)
     for s in range(S):
         ref = np.asarray(hp.alm2map(alms[s], nside, pol=False, precision="fp64"), dtype=np.float64)
         e_plain, e_form = relerr(out[-1][s], ref), relerr(out[flush][s], ref)

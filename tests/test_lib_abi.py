@@ -101,6 +101,8 @@ def test_tuning_default_is_checked_and_restorable():
     assert L.ga_set_tuning(99, 0) == _lib.GA_ERR_INVALID
     # accumulation forms: one chain (-1), two chunks (-2, -3), master sums (n > 0)
     assert L.ga_set_tuning(_lib.TUNE_F16_FLUSH, -4) == _lib.GA_ERR_INVALID
+    assert L.ga_set_tuning(_lib.TUNE_CHUNK_MIN, -1) == _lib.GA_ERR_INVALID
+    assert L.ga_set_tuning(_lib.TUNE_M_GROUP, 2) == _lib.GA_ERR_INVALID
     assert L.ga_plan_set_tuning(None, _lib.TUNE_MFMA_PATH, 0) == _lib.GA_ERR_INVALID
     assert L.ga_plan_get_tuning(None, _lib.TUNE_MFMA_PATH, ctypes.byref(v)) == _lib.GA_ERR_INVALID
 

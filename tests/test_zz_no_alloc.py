@@ -63,9 +63,16 @@ def test_no_alloc_synthesis_matches_workspace_entry_point():
         assert plan.workspace_bytes(S, 4) <= cap.value
         alm = _alm_batch(lmax, S, seed)
         epi = _epilogues(S)
-        ref = _with_workspace(plan, S, alm, epi)
-        got = _no_alloc(plan, S, alm, epi)
-        torch.cuda.synchronize()
+        # split the l-ranges into two chunks even at these sizes, so that the
+        # parking of both forms (its own region here, the ring buffer in the
+        # no-alloc call) is compared
+        plan.set_tuning(_lib.TUNE_CHUNK_MIN, 2)
+        try:
+            ref = _with_workspace(plan, S, alm, epi)
+            got = _no_alloc(plan, S, alm, epi)
+            torch.cuda.synchronize()
+        finally:
+            plan.set_tuning(_lib.TUNE_CHUNK_MIN, 0)
         assert torch.isfinite(ref).all()
         assert torch.equal(got, ref)
 
