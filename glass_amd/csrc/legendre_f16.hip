@@ -372,6 +372,14 @@ __device__ __forceinline__ void park_tile(f32x16 (&acc)[2][2][NT], int m, int lm
   }
 }
 
+// The parking slot of a wave (in float4): one per (m, tile of the m, wave),
+// numbered m-major.  Only m whose l-range reaches the split threshold park,
+// and those are m < lmax + 1 - 32 (chunk_min - 1): the region holds that many
+// m, not all of them.
+__device__ __forceinline__ int64_t park_slot(int m, int rem, int G, int wave, int nt) {
+  return ((int64_t(m) * G + rem) * 4 + wave) * (16 * nt * 64);
+}
+
 // Two-chunk parking in a region of its own: the wave's accumulators as 16 NT
 // float4 per lane, lane-major (every store and load a whole 1 KiB per wave),
 // in the wave's slot of the region (one slot per wave of the grid, 16 NT
@@ -638,7 +646,7 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   const bool two = !FLUSH && nchunk == 2 && nsl - s >= g.chunk_min;
   run_slabs(two ? s + (nsl - s) / 2 : nsl);
   if (two) {
-    if (SEP) park_lanes<NT, false>(acc, g.part + (int64_t(b) * 4 + wave) * (16 * NT * 64) + lane);
+    if (SEP) park_lanes<NT, false>(acc, g.part + park_slot(m, rem, G, wave, NT) + lane);
     else park_tile<NT, false>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
@@ -647,7 +655,7 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
     run_slabs(nsl);
-    if (SEP) park_lanes<NT, true>(acc, g.part + (int64_t(b) * 4 + wave) * (16 * NT * 64) + lane);
+    if (SEP) park_lanes<NT, true>(acc, g.part + park_slot(m, rem, G, wave, NT) + lane);
     else park_tile<NT, true>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
   }
   if (FLUSH) {

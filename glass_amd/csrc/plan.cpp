@@ -564,14 +564,16 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch, const ga_tuning &tu) {
     // of the equator row needs a place of its own: (lmax + 1) x 2 nbatch
     w.p_bytes = up(size_t(p->lmax + 1) * size_t(2 * nbatch) * 4);
   } else if (acc_form == -3) {
-    // two chunks park the first in a slot per wave of the two-tile launch
-    // grid (8 m-groups x ceil((lmax + 1) / 8) x ring groups x 64-column
-    // groups, 4 waves each): 32 float4 per lane; the one-tile grid needs at
-    // most as much
-    // (m slots per XCD: the larger count, that of runs of 4 consecutive m)
-    const size_t nm8 = 4 * (size_t(p->lmax + 1 + 31) / 32), nrg = size_t(p->npairs + 255) / 256;
+    // two chunks park the first in a slot per (m, ring group, 64-column
+    // group, wave) of the two-tile launch grid: 32 float4 per lane; the
+    // one-tile grid needs at most as much.  Only the m whose l-range of
+    // ceil((lmax - m + 1) / 32) slabs reaches the split threshold park:
+    // m < lmax + 1 - 32 (chunk_min - 1)
+    const int64_t cmin = tu.chunk_min > 0 ? tu.chunk_min : kTwoChunkMinSlabs;
+    const int64_t mpark = std::max<int64_t>(0, int64_t(p->lmax) + 1 - 32 * (cmin - 1));
+    const size_t nrg = size_t(p->npairs + 255) / 256;
     const size_t ncg = size_t(2 * nbatch + 63) / 64;
-    w.p_bytes = up(8 * nm8 * nrg * ncg * 4 * 64 * 32 * 16);
+    w.p_bytes = up(size_t(mpark) * nrg * ncg * 4 * 64 * 32 * 16);
   }
   w.total = w.p_off + w.p_bytes;
   return w;
