@@ -142,6 +142,18 @@ struct ga_plan {
   // first slab of m (lmax + 2 entries, slab_off[lmax + 1] = total)
   int64_t *d_slab_off = nullptr;
   int64_t nslab = 0;
+  // Where the split-FP16 synthesis starts each wave's slab loop.  A wave
+  // runs 64 consecutive ring pairs (group g = rp / 64) for one m; the leading
+  // slabs in which every lane stays below the FP16 range contribute nothing.
+  // Per (m, g): the first slab that is not dead for the whole wave
+  // (d_f16_s0[m * f16_ngroups + g]); per (m, ring pair, padded to 64 per
+  // group): the recurrence state (y_{l-2}, y_{l-1}) and scale count there
+  // (d_f16_st / d_f16_k[m * 64 f16_ngroups + rp]).  Built on the device at
+  // plan creation by the same code the contraction ran before its loop.
+  int32_t f16_ngroups = 0;
+  int32_t *d_f16_s0 = nullptr;
+  double2 *d_f16_st = nullptr;
+  int32_t *d_f16_k = nullptr;
   // spin recurrence coefficients (A, B, C) per (l, m) for s = 1, 2 (lazily built)
   double *d_spin[3] = {nullptr, nullptr, nullptr};
   // spin seed constants per m: [+s: (signed mantissa, exponent)] then [-s: ...]

@@ -33,6 +33,9 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,
                                  const int32_t *colexp, hipStream_t s, int nt, int flush,
                                  void *park, int mgroup, int chunk_min);
+// fills the plan's start table of the split-FP16 synthesis (d_f16_s0,
+// d_f16_st, d_f16_k, allocated by the caller) from its recurrence tables
+int launch_legendre_f16_start(const ga_plan *p, hipStream_t s);
 // the batched Legendre stage on the path chosen by the tuning set tu (a
 // snapshot of the plan's, the same for the preparation and the contraction
 // of one synthesis); ws is the FP32 workspace with F at its start

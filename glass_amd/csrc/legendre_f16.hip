@@ -263,8 +263,13 @@ __device__ __forceinline__ void slab_values(const double *__restrict__ cb, doubl
   }
 }
 
-// The dead leading slabs of a wave (see the synthesis kernel); returns the
-// first slab to run in full.  A slab is dead when every lane either starts it
+// The dead leading slabs of a wave: only the recurrence advances, without
+// values, splitting, B loads or MFMAs, and with one rescaling check per slab
+// (a slab grows |lambda| by far less than the 2^700 above the threshold;
+// power-of-two rescaling is exact, so the state matches the per-group checks
+// of slab_values bit for bit).  The slab in which some lane reaches FP16
+// range is undone, to be run again in full; returns its index (the first
+// slab to run in full).  A slab is dead when every lane either starts it
 // scaled (k < 0: all its values are below 2^-300) or keeps its carried value
 // (y 2^lam_exp) below 2^-25 in magnitude throughout, where both FP16 halves of the split are exactly 0 (the
 // high half rounds toward zero below 2^-24, the low half to nearest): the
@@ -307,6 +312,42 @@ __device__ __forceinline__ int skip_dead_slabs(double (*cb)[kSlab],
     ++s;
   }
   return s;
+}
+
+// The start table of a plan (ga_plan::d_f16_s0 / d_f16_st / d_f16_k): one
+// wave per (m, group of 64 ring pairs), the synthesis kernel's preamble --
+// the seed, the coefficient double buffer, skip_dead_slabs -- run once per
+// plan instead of in every synthesis.  Lanes past the last ring pair repeat
+// it, as in the synthesis.  The state is the one the synthesis loop would
+// start from (the same operations in the same order), so it replaces the
+// preamble there without changing a bit of the result.
+__global__ __launch_bounds__(256) void legendre_f16_start_kernel(
+    int lmax, int npairs, int ngroups, const double *__restrict__ z,
+    const double *__restrict__ sth, const double *__restrict__ gam,
+    const double *__restrict__ cm, int lam_exp, int32_t *__restrict__ s0tab,
+    double2 *__restrict__ st, int32_t *__restrict__ stk) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int grp = int(blockIdx.x) * 4 + wave, m = int(blockIdx.y);
+  if (grp >= ngroups) return;  // wave-uniform
+  const int rp = grp * 64 + lane;
+  const int rpc = rp < npairs ? rp : npairs - 1;
+  const double x = z[rpc];
+  double p0, p1 = 0.0;
+  int k;
+  legendre_seed(ldexp(cm[m], lam_exp), sth[rpc], m, p0, k);
+  p0 = -p0;
+  const double *__restrict__ co = gam + hp_index(lmax, m, m);
+  __shared__ __align__(16) double cbuf[4][2][kSlab];
+  const int lastc = lmax - m;
+  double cpre = co[min(lane & (kSlab - 1), lastc)];
+  if (lane < kSlab) cbuf[wave][0][lane] = cpre;
+  __builtin_amdgcn_wave_barrier();
+  const int nsl = int(nslab_of(lmax, m));
+  const int s = skip_dead_slabs(cbuf[wave], co, lastc, lane, x, m, lmax, nsl, p0, p1, k, cpre);
+  const int64_t i = int64_t(m) * (int64_t(ngroups) * 64) + rp;
+  st[i] = make_double2(p0, p1);
+  stk[i] = k;
+  if (lane == 0) s0tab[int64_t(m) * ngroups + grp] = s;
 }
 
 // Two-chunk parking in the ring buffer.  The first chunk's raw accumulators
@@ -427,6 +468,10 @@ struct f16_geom {
   float *F;
   float *eqo;     // two chunks parked in F: the equator row's O half
   float4 *part;   // two chunks parked in a region of their own (sep)
+  const int32_t *s0tab;  // the plan's start table (ga_plan::d_f16_*)
+  const double2 *st;
+  const int32_t *stk;
+  int ngroups;           // groups of 64 ring pairs
   int sep;
   int lmax, npairs, nrings, ncols, nct, nrg, ncg, lam_exp, flush, nchunk, nt, blk_begin;
   int mgroup;     // consecutive m per XCD turn (1 or 4)
@@ -517,9 +562,7 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   const int lmax = g.lmax, npairs = g.npairs, nrings = g.nrings, ncols = g.ncols, nct = g.nct,
             nrg = g.nrg, ncg = g.ncg, lam_exp = g.lam_exp, flush = g.flush, nchunk = g.nchunk;
   const double *__restrict__ z = g.z;
-  const double *__restrict__ sth = g.sth;
   const double *__restrict__ gam = g.gam;
-  const double *__restrict__ cm = g.cm;
   const int32_t *__restrict__ ring_pnum = g.ring_pnum;
   const int32_t *__restrict__ ring_pden = g.ring_pden;
   const int64_t *__restrict__ slab_off = g.slab_off;
@@ -544,11 +587,23 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   const int rpc = rp < npairs ? rp : npairs - 1;
   const double x = z[rpc];
   // the normalised recurrence y_l = (gam_l x) y_{l-1} - y_{l-2}: one multiply
-  // and one FMA per step, the FMA alone on the dependency chain
-  double p0, p1 = 0.0;  // y_{l-2}, y_{l-1} (times 2^lam_exp); gam at l = m is 0,
-  int k;                // so one step from (-seed, 0) yields y_m = lambda_mm
-  legendre_seed(ldexp(cm[m], lam_exp), sth[rpc], m, p0, k);
-  p0 = -p0;
+  // and one FMA per step, the FMA alone on the dependency chain.  It starts
+  // at l = m from (-seed, 0) (gam at l = m is 0, so one step yields
+  // y_m = lambda_mm); the leading slabs in which every lane stays below the
+  // FP16 range of the split (near the poles at large m) contribute nothing,
+  // and the plan's start table holds, per wave, the first slab that does not
+  // (skip_dead_slabs, run at plan creation) and each lane's state there.
+  double p0, p1;  // y_{l-2}, y_{l-1} (times 2^lam_exp)
+  int k;          // scale count (value = p 2^(600 k))
+  int s;          // first slab run in full
+  {
+    const int64_t si = int64_t(m) * (int64_t(g.ngroups) * 64) + rp;
+    const double2 st = g.st[si];
+    p0 = st.x;
+    p1 = st.y;
+    k = g.stk[si];
+    s = __builtin_amdgcn_readfirstlane(g.s0tab[int64_t(m) * g.ngroups + (rp0 >> 6)]);
+  }
 
   const int64_t base = hp_index(lmax, m, m);
   const double *__restrict__ co = gam + base;
@@ -581,18 +636,10 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   // wave touches its buffers, so no workgroup barrier is needed.
   __shared__ __align__(16) double cbuf[4][2][kSlab];
   const int lastc = lmax - m;  // last valid row index of this m
-  double cpre = co[min(lane & (kSlab - 1), lastc)];
-  if (lane < kSlab) cbuf[wave][0][lane] = cpre;
+  double cpre = co[min(kSlab * s + (lane & (kSlab - 1)), lastc)];
+  if (lane < kSlab) cbuf[wave][s & 1][lane] = cpre;
   __builtin_amdgcn_wave_barrier();
 
-  // Leading slabs in which every lane stays below the FP16 range of the split
-  // (near the poles at large m) contribute nothing: only the recurrence
-  // advances, without values, splitting, B loads or MFMAs, and with one
-  // rescaling check per slab (a slab grows |lambda| by far less than the
-  // 2^700 above the threshold; power-of-two rescaling is exact, so the state
-  // matches the per-group checks bit for bit).  The slab in which some lane
-  // reaches FP16 range is undone and run again by the full loop.
-  int s = skip_dead_slabs(cbuf[wave], co, lastc, lane, x, m, lmax, nsl, p0, p1, k, cpre);
   // the slab loop from s to end (accumulating into acc)
   auto run_slabs = [&](const int end) {
     for (; s < end; ++s) {
@@ -879,6 +926,10 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.sth = p->d_sth;
   g.gam = p->d_gam;
   g.cm = p->d_cm;
+  g.s0tab = p->d_f16_s0;
+  g.st = p->d_f16_st;
+  g.stk = p->d_f16_k;
+  g.ngroups = p->f16_ngroups;
   g.ring_pnum = p->d_ring_pnum;
   g.ring_pden = p->d_ring_pden;
   g.slab_off = p->d_slab_off;
@@ -906,6 +957,19 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   const int slots = g.mgroup == 4 ? 4 * ((p->lmax + 1 + 31) / 32) : (p->lmax + 1 + 7) / 8;
   *nblk = int64_t(8) * slots * g.nrg * g.ncg;
   return g;
+}
+
+int launch_legendre_f16_start(const ga_plan *p, hipStream_t stream) {
+  GA_REQUIRE(p->d_f16_s0 && p->d_f16_st && p->d_f16_k, GA_ERR_INVALID,
+             "start table not allocated");
+  const int ngroups = p->f16_ngroups;
+  GA_REQUIRE(ngroups * 64 >= p->npairs, GA_ERR_INVALID, "start table groups %d for %d ring pairs",
+             ngroups, p->npairs);
+  hipLaunchKernelGGL(legendre_f16_start_kernel, dim3(unsigned((ngroups + 3) / 4), unsigned(p->lmax + 1)),
+                     dim3(256), 0, stream, p->lmax, int(p->npairs), ngroups, p->d_z, p->d_sth,
+                     p->d_gam, p->d_cm, p->lam_exp16, p->d_f16_s0, p->d_f16_st, p->d_f16_k);
+  GA_LAUNCH_CHECK();
+  return GA_OK;
 }
 
 int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *bws,

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "common.h"
+#include "kernels.h"
 
 namespace ga {
 
@@ -109,6 +110,18 @@ void host_fft(std::vector<std::complex<double>> &a) {
       }
     }
   }
+}
+
+template <typename T>
+int alloc_table(T **dst, size_t bytes, size_t *acc) {
+  hipError_t e = hipMalloc((void **)dst, bytes);
+  if (e != hipSuccess) {
+    *dst = nullptr;
+    set_error("hipMalloc(%zu) for plan tables failed: %s", bytes, hipGetErrorString(e));
+    return GA_ERR_NOMEM;
+  }
+  *acc += bytes;
+  return GA_OK;
 }
 
 template <typename T>
@@ -382,6 +395,22 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
     so[size_t(lmax + 1)] = ns;
     p->nslab = ns;
     if ((rc = upload(&p->d_slab_off, so.data(), so.size() * 8, &p->table_bytes))) return fail(rc);
+    // where each wave of the split-FP16 synthesis starts its slab loop, and
+    // the recurrence state there (computed on the device from the tables
+    // above, by the same code the synthesis would run)
+    p->f16_ngroups = (p->npairs + 63) / 64;
+    const size_t nst = size_t(lmax + 1) * size_t(p->f16_ngroups) * 64;
+    if ((rc = alloc_table(&p->d_f16_s0, size_t(lmax + 1) * size_t(p->f16_ngroups) * 4,
+                          &p->table_bytes)) ||
+        (rc = alloc_table(&p->d_f16_st, nst * sizeof(double2), &p->table_bytes)) ||
+        (rc = alloc_table(&p->d_f16_k, nst * 4, &p->table_bytes)))
+      return fail(rc);
+    if ((rc = launch_legendre_f16_start(p, nullptr))) return fail(rc);
+    const hipError_t se = hipStreamSynchronize(nullptr);
+    if (se != hipSuccess) {
+      set_error("building the start table failed: %s", hipGetErrorString(se));
+      return fail(GA_ERR_HIP);
+    }
   }
 
   // ---- ring FFT tables: every ring has n = 4 r pixels
@@ -514,7 +543,7 @@ extern "C" int ga_plan_destroy(ga_plan *p) {
                   p->d_spin_seed[0], p->d_spin_seed[1], p->d_spin_seed[2], p->d_blu_off,
                   p->d_blu64, p->d_blu32, p->d_tw64, p->d_tw32, p->d_rings_pow2,
                   p->d_rings_blu, p->d_rings_wide, p->d_rings_narrow, p->d_rings_bwide,
-                  p->d_rings_bnarrow};
+                  p->d_rings_bnarrow, p->d_f16_s0, p->d_f16_st, p->d_f16_k};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
