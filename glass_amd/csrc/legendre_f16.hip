@@ -36,6 +36,8 @@
 // 32 x 16 A operand (lane L: ring L % 32, k = 8 (L / 32) .. + 7).  B comes
 // pre-split and pre-swizzled (one 16-byte load per lane for the high and one
 // for the low part), prepared once per batch by the two small kernels below.
+#include <type_traits>
+
 #include "common.h"
 #include "device.h"
 #include "f16.h"
@@ -483,13 +485,23 @@ struct f16_group {
   int ngroup;
 };
 
+// the value of the partner lane of a (re, im) pair: lanes 2c <-> 2c + 1
+__device__ __forceinline__ float pair_swap(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+
 // Epilogue of a wave's accumulators (as in the FP32 stage): D layout of
 // 32x32: lane L, element e -> row 8 (e/4) + 4 (L/32) + e%4, column L % 32.
 // Unscale by 2^-(8 + shift of the column), north = E + O, south = E - O,
 // times the ring phase e^{i m phi0} (each lane computes its own ring's phase
 // in FP64; rows fetch it by shuffle; the (re, im) columns of a shell sit in
-// lanes (2c, 2c + 1)).  Ring buffer layout: real column c of ring r at
+// lanes (2c, 2c + 1), exchanged by DPP).  Ring buffer layout: real column c
+// of ring r at
 //   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8).
+// Addresses: per element a wave-uniform ring base (the row's ring for the
+// lower half-wave) plus a per-lane 32-bit offset fixed for the whole tile
+// (the column part, and 4 rings for the other half-wave), so that a store
+// costs no address arithmetic.
 template <int NT>
 __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m, int lmax,
                                            int npairs, int nrings, int ncols, int colbase, int rp0,
@@ -506,41 +518,62 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
     phc = float(pc);
     phs = float(ps);
   }
+  // rotation by (c, s): re' = re c - im s, im' = im c + re s, i.e. own value
+  // times c plus the partner's times -s (real lane) or +s (imaginary lane);
+  // at m = 0 the imaginary part is 0 (both factors 0 in an imaginary lane)
+  const bool is_im = (col_in & 1) != 0;
+  const float zf = (m == 0 && is_im) ? 0.0f : 1.0f;
+  const float sg = is_im ? zf : -zf;
+  const int64_t rstride = int64_t((ncols + 7) / 8) * (lmax + 1) * 8;  // floats per ring
   float fac[NT];
+  bool cok[NT];
+  uint32_t noff[NT], soff[NT];
 #pragma unroll
   for (int ct = 0; ct < NT; ++ct) {
     const int col = colbase + ct * 32 + col_in;
-    fac[ct] = col < ncols ? ldexpf(1.0f, -(lam_exp + colexp[col])) : 0.0f;
+    cok[ct] = col < ncols;
+    fac[ct] = cok[ct] ? ldexpf(1.0f, -(lam_exp + colexp[col])) : 0.0f;
+    // byte offsets (< 2^32: the column part of a ring plus 4 rings)
+    const uint32_t cofs = uint32_t(((col >> 3) * (lmax + 1) + m) * 8 + (col & 7)) * 4u;
+    noff[ct] = cofs + uint32_t(half * 4 * rstride) * 4u;
+    soff[ct] = cofs + uint32_t((1 - half) * 4 * rstride) * 4u;
   }
-  const int ngrp = (ncols + 7) / 8;
-  const bool is_im = (col_in & 1) != 0;
+  const int rpu = __builtin_amdgcn_readfirstlane(rp0);  // wave-uniform: scalar ring bases
+  // every row has its north and south ring and every lane its columns: the
+  // stores need no guard
+  const bool full = rpu + 63 < npairs - 1 && wave_all(cok[0] && cok[NT - 1]);
+  auto tile = [&](auto guarded) {
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < 2; ++t) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int row = 32 * t + 8 * (e >> 2) + 4 * half + (e & 3);
-      const float c = __shfl(phc, row, 64), sn = __shfl(phs, row, 64);
-      const int rpo = rp0 + row;
+      for (int e = 0; e < 16; ++e) {
+        const int r0 = 32 * t + 8 * (e >> 2) + (e & 3);  // row of the lower half-wave
+        const int row = r0 + 4 * half;
+        const float cz = __shfl(phc, row, 64) * zf, sz = __shfl(phs, row, 64) * sg;
+        // north ring rpu + r0 (+ 4 in the upper half-wave); south ring
+        // nrings - 1 - rpu - r0 - 4 (+ 4 in the lower half-wave)
+        char *const fn = reinterpret_cast<char *>(F + int64_t(rpu + r0) * rstride);
+        char *const fs = reinterpret_cast<char *>(F + int64_t(nrings - 5 - rpu - r0) * rstride);
 #pragma unroll
-      for (int ct = 0; ct < NT; ++ct) {
-        const int col = colbase + ct * 32 + col_in;
-        const float ev = acc[0][t][ct][e] * fac[ct], ov = acc[1][t][ct][e] * fac[ct];
-        const float nv = ev + ov, sv = ev - ov;
-        const float no = __shfl_xor(nv, 1, 64), so = __shfl_xor(sv, 1, 64);
-        // re' = re c - im s,  im' = re s + im c
-        const float nrot = is_im ? (no * sn + nv * c) : (nv * c - no * sn);
-        const float srot = is_im ? (so * sn + sv * c) : (sv * c - so * sn);
-        if (rpo < npairs && col < ncols) {
-          const float nrz = (m == 0 && is_im) ? 0.0f : nrot;
-          const float srz = (m == 0 && is_im) ? 0.0f : srot;
-          const int64_t gofs = (int64_t(col >> 3) * (lmax + 1) + m) * 8 + (col & 7);
-          F[int64_t(rpo) * ngrp * (lmax + 1) * 8 + gofs] = nrz;
-          if (rpo < npairs - 1)
-            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * 8 + gofs] = srz;
+        for (int ct = 0; ct < NT; ++ct) {
+          const float ev = acc[0][t][ct][e] * fac[ct], ov = acc[1][t][ct][e] * fac[ct];
+          const float nv = ev + ov, sv = ev - ov;
+          const float nrot = fmaf(pair_swap(nv), sz, nv * cz);
+          const float srot = fmaf(pair_swap(sv), sz, sv * cz);
+          if (!decltype(guarded)::value) {
+            *reinterpret_cast<float *>(fn + noff[ct]) = nrot;
+            *reinterpret_cast<float *>(fs + soff[ct]) = srot;
+          } else {
+            const int rpo = rpu + row;
+            if (cok[ct] && rpo < npairs) *reinterpret_cast<float *>(fn + noff[ct]) = nrot;
+            if (cok[ct] && rpo < npairs - 1) *reinterpret_cast<float *>(fs + soff[ct]) = srot;
+          }
         }
       }
     }
-  }
+  };
+  if (full) tile(std::false_type{});
+  else tile(std::true_type{});
 }
 
 // FLUSH: every `flush` slabs the accumulators are added into FP32 master sums
