@@ -1198,6 +1198,133 @@ __device__ __forceinline__ void fold_split_wide(const cplx<float> *__restrict__ 
   }
 }
 
+// Paired fold of the wide rings.  Bins k1 + r k2 and (r - k1) + r k2 need
+// the same coefficients: n - (k1 + r k2) = (r - k1) + r (3 - k2), so the
+// direct terms of one are the conjugate terms of the other.  A thread folds
+// both from one set of loads, Fa[j] = F_{a + r j} and Fb[j] = F_{b + r j}
+// (zero above lmax; b = r - a):
+//   G_{a + r k2} = Fa[k2] + conj(Fb[3 - k2]),  G_{b + r k2} = Fb[k2] + conj(Fa[3 - k2])
+// (direct term first, as in wide_bin_first / wide_bin_second).  Thread t owns
+// q = t + i NT (i < C / 2, q < (r + 1) / 2) and the outputs k1 = q and
+// r - q, in slots 2 i and 2 i + 1; q = 0 owns k1 = 0 (whose conjugate terms
+// are F_{r (4 - k2)}: b = r) and, for even r, the self-paired k1 = r / 2.
+// Every coefficient of the ring is loaded once, where the unpaired fold
+// loads most twice.  The loads of B values of q are issued together: j = 0,
+// 1 (every term when lmax < 2 r, all but F_{2r} at lmax = 2 r), then, only
+// where lmax >= 2 r, j = 2, 3.  sink(slot, k1, valid, h) takes the folded
+// bins h[k2] of output k1, the two shells of the unit packed as one complex
+// transform.
+__device__ __forceinline__ int wide_pair_k1(int q, int r, int second) {
+  return second ? (q ? r - q : r / 2) : q;
+}
+__device__ __forceinline__ bool wide_pair_valid(int q, int r, int second) {
+  return q < (r + 1) / 2 && (!second || q || !(r & 1));
+}
+
+template <int NT, int C, int B, int MODE, typename Sink>
+__device__ __forceinline__ void fold_wide_pairs(const cplx<float> *__restrict__ Fu, int G, int r,
+                                                int lmax, Sink &&sink) {
+  static_assert(C % 2 == 0 && (C / 2) % B == 0, "paired fold: C / 2 q values in batches of B");
+  const int t = threadIdx.x;
+  constexpr int P = C / 2;
+  const bool far = lmax >= 2 * r;  // terms at j >= 2 (block-uniform)
+#pragma unroll
+  for (int i0 = 0; i0 < P; i0 += B) {
+    cplx<float> f[B][2][4][2];  // [q][side a / b][j][shell of the pair]
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int q = t + (i0 + u) * NT;
+      const bool okq = q < (r + 1) / 2;
+      const int ab[2] = {q, q ? r - q : r};
+#pragma unroll
+      for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          f[u][sd][j][0] = f[u][sd][j][1] = cplx<float>{0.0f, 0.0f};
+          const int mm = ab[sd] + r * j;
+          if (j < 2 && okq && mm <= lmax)
+            load_pair<MODE>(Fu + int64_t(mm) * G, f[u][sd][j][0], f[u][sd][j][1]);
+        }
+    }
+    if (far) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int q = t + (i0 + u) * NT;
+        const bool okq = q < (r + 1) / 2;
+        const int ab[2] = {q, q ? r - q : r};
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+          for (int j = 2; j < 4; ++j) {
+            const int mm = ab[sd] + r * j;
+            if (okq && mm <= lmax)
+              load_pair<MODE>(Fu + int64_t(mm) * G, f[u][sd][j][0], f[u][sd][j][1]);
+          }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int q = t + (i0 + u) * NT;
+#pragma unroll
+      for (int second = 0; second < 2; ++second) {
+        const bool valid = wide_pair_valid(q, r, second);
+        const int k1 = wide_pair_k1(q, r, second);
+        cplx<float> g[4][2];
+        if (second && q == 0) {
+          // k1 = r / 2 (even r): its own coefficients F_{r/2 + r j}, paired with itself
+          cplx<float> c[4][2];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            c[j][0] = c[j][1] = cplx<float>{0.0f, 0.0f};
+            const int mm = r / 2 + r * j;
+            if (valid && mm <= lmax) load_pair<MODE>(Fu + int64_t(mm) * G, c[j][0], c[j][1]);
+          }
+#pragma unroll
+          for (int k2 = 0; k2 < 4; ++k2)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) g[k2][h] = cadd(c[k2][h], cconj(c[3 - k2][h]));
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < 4; ++k2)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+              g[k2][h] = cadd(f[u][second][k2][h], cconj(f[u][1 - second][3 - k2][h]));
+        }
+        cplx<float> hh[4];
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2)
+          hh[k2] = cplx<float>{g[k2][0].re - g[k2][1].im, g[k2][0].im + g[k2][1].re};
+        sink(2 * (i0 + u) + second, k1, valid, hh);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // one batch of loads in flight
+  }
+}
+
+// the paired fold of a wide power-of-two ring and its radix-4 split:
+// sub-transform inputs 0 and 1 to LDS (bit-reversed), 2 and 3 to park
+template <int NT, int C, int MODE>
+__device__ __forceinline__ void fold_split_wide_paired(const cplx<float> *__restrict__ Fu, int G,
+                                                       int r, int lmax,
+                                                       const float2 *__restrict__ tw, int ltw,
+                                                       int ldx, cplx<float> *X,
+                                                       cplx<float> (&park)[C][2]) {
+  const int n = 4 * r, ln = ilog2(n), lbits = ilog2(r);
+  fold_wide_pairs<NT, C, (C / 2 >= 2 ? 2 : 1), MODE>(
+      Fu, G, r, lmax, [&](int slot, int k1, bool, const cplx<float> (&hh)[4]) {
+        const cplx<float> w1 = twid(tw, k1 << (ltw - ln), 1);
+        const cplx<float> w2 = cmul(w1, w1);
+        const cplx<float> w3 = cmul(w2, w1);
+        const cplx<float> s02 = cadd(hh[0], hh[2]), d02 = csub(hh[0], hh[2]);
+        const cplx<float> s13 = cadd(hh[1], hh[3]), d13 = cmuli(csub(hh[1], hh[3]));
+        const int pos = int(__brev(unsigned(k1)) >> (32 - lbits));
+        X[pad(pos)] = cadd(s02, s13);
+        X[ldx + pad(pos)] = cmul(cadd(d02, d13), w1);
+        park[slot][0] = cmul(csub(s02, s13), w2);
+        park[slot][1] = cmul(csub(d02, d13), w3);
+      });
+}
+
 template <int NT, int C>
 __global__ __launch_bounds__(NT, 4) void ring_synth_wide_kernel(
     int lmax, const int32_t *__restrict__ ring_list, int nlist,
@@ -1218,13 +1345,23 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_kernel(
   const int ltw = ilog2(twlen), lbits = ilog2(r);
 
   // ---- fold + radix-4 split; sub-transform inputs 0, 1 to LDS (bit-reversed)
+  // (one k1 per thread, C = 1: the unpaired fold)
   cplx<float> park[C][2];
-  if (!two)
-    fold_split_wide<NT, C, 0>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
-  else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
-    fold_split_wide<NT, C, 2>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
-  else
-    fold_split_wide<NT, C, 1>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+  if constexpr (C % 2 == 0) {
+    if (!two)
+      fold_split_wide_paired<NT, C, 0>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+    else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
+      fold_split_wide_paired<NT, C, 2>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+    else
+      fold_split_wide_paired<NT, C, 1>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+  } else {
+    if (!two)
+      fold_split_wide<NT, C, 0>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+    else if (G % 2 == 0)
+      fold_split_wide<NT, C, 2>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+    else
+      fold_split_wide<NT, C, 1>(Fu, G, r, lmax, tw, ltw, ldx, X, park);
+  }
   __syncthreads();
   fused_fft<NT, true, +1>(X, ldx, 2, r, tw, twlen);
   // ---- results of 0, 1 to registers; inputs of 2, 3 to LDS
@@ -1238,7 +1375,8 @@ __global__ __launch_bounds__(NT, 4) void ring_synth_wide_kernel(
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < C; ++i) {
-    const int pos = int(__brev(unsigned(t + i * NT)) >> (32 - lbits));
+    const int k1 = C % 2 == 0 ? wide_pair_k1(t + (i / 2) * NT, r, i & 1) : t + i * NT;
+    const int pos = int(__brev(unsigned(k1)) >> (32 - lbits));
     X[pad(pos)] = park[i][0];
     X[ldx + pad(pos)] = park[i][1];
   }
@@ -1400,6 +1538,18 @@ __global__ __launch_bounds__(NT, (NT == 256 && C == 4) ? 3 : 4) void ring_synth_
     __syncthreads();
     split_from_lds<NT, C>(X, ldf, r, chirp, y);
     __syncthreads();
+  } else if constexpr (C % 2 == 0) {
+    // paired fold (slot c holds output k1 = blu_slot_k1(c))
+    const int n = 4 * r;
+    auto sink = [&](int slot, int k1, bool valid, const cplx<float> (&hh)[4]) {
+      split_chirp(hh, valid ? k1 : 0, n, chirp, y[slot]);
+    };
+    if (!two)
+      fold_wide_pairs<NT, C, 1, 0>(Fu, G, r, lmax, sink);
+    else if (G % 2 == 0)  // the unit's pair is 16-byte aligned
+      fold_wide_pairs<NT, C, 1, 2>(Fu, G, r, lmax, sink);
+    else
+      fold_wide_pairs<NT, C, 1, 1>(Fu, G, r, lmax, sink);
   } else {
     if (!two)
       fold_split_wide_blu<NT, C, 0>(Fu, G, r, lmax, chirp, y);
@@ -1408,6 +1558,16 @@ __global__ __launch_bounds__(NT, (NT == 256 && C == 4) ? 3 : 4) void ring_synth_
     else
       fold_split_wide_blu<NT, C, 1>(Fu, G, r, lmax, chirp, y);
   }
+  // input position of slot c: k1 = t + c NT, or the paired fold's output
+  auto in_k1 = [&](int c, bool &valid) {
+    if (!LDSFOLD && C % 2 == 0) {
+      const int q = t + (c / 2) * NT;
+      valid = wide_pair_valid(q, r, c & 1);
+      return wide_pair_k1(q, r, c & 1);
+    }
+    valid = t + c * NT < r;
+    return t + c * NT;
+  };
 
   // ---- the four convolutions, A at a time; results back into y
 #pragma unroll
@@ -1416,8 +1576,9 @@ __global__ __launch_bounds__(NT, (NT == 256 && C == 4) ? 3 : 4) void ring_synth_
     for (int a = 0; a < A; ++a)
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        const int k1 = t + c * NT;
-        if (k1 < r) X[a * ldx + pad(k1)] = y[c][q0 + a];
+        bool valid;
+        const int k1 = in_k1(c, valid);
+        if (valid) X[a * ldx + pad(k1)] = y[c][q0 + a];
       }
     for (int e = t; e < (L - r) * A; e += NT) {
       const int a = e / (L - r), j = r + e - a * (L - r);
