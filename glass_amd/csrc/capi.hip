@@ -25,7 +25,7 @@ __global__ void weight_pixels_kernel(int64_t n, int64_t npix, const T *__restric
 }
 
 // same size as ga_workspace_size: fp32 ring buffers hold whole groups of
-// kMfmaGroup shells
+// ring_group(nbatch) shells
 size_t ring_bytes(const ga_plan *p, int nbatch, int rb) {
   size_t b = 0;
   (void)ga_workspace_size(p, nbatch, rb, &b);
@@ -83,7 +83,7 @@ extern "C" int ga_alm2map_batch_f32(const ga_plan *p, int32_t S, const float *al
   hipStream_t s = (hipStream_t)stream;
   int rc = launch_legendre_synth_batch(p, S, alm, ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)ws, maps, epi, s);
+  return launch_ring_synth_f32(p, S, ring_group(S), (const float *)ws, maps, epi, s);
 }
 
 // ---- fixed device scratch of ga_alm2map_batch_f32_noalloc -------------------
@@ -148,7 +148,7 @@ extern "C" int ga_alm2map_batch_f32_noalloc(const ga_plan *p, int32_t S, const f
   hipStream_t s = (hipStream_t)stream;
   rc = launch_legendre_synth_batch(p, tu, S, alm, ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)ws, maps, epi, s);
+  return launch_ring_synth_f32(p, S, ring_group(S), (const float *)ws, maps, epi, s);
 }
 
 extern "C" int ga_legendre_synth_batch_f32(const ga_plan *p, int32_t S, const float *alm,
@@ -210,7 +210,7 @@ extern "C" int ga_ring_synth_batch_f32(const ga_plan *p, int32_t S, const float 
                                        const ga_epilogue *epi, void *stream) {
   GA_PLAN_OK(p);
   GA_REQUIRE(S >= 1 && F && maps, GA_ERR_INVALID, "invalid argument");
-  return launch_ring_synth_f32(p, S, kMfmaGroup, F, maps, epi, (hipStream_t)stream);
+  return launch_ring_synth_f32(p, S, ring_group(S), F, maps, epi, (hipStream_t)stream);
 }
 
 // pixw: per-pixel quadrature weights (npix doubles, summing to 4 pi) in place
@@ -243,12 +243,12 @@ static int map2alm_impl(const ga_plan *p, int S, const T *map, T *alm, int niter
         rc = launch_legendre_synth_batch(p, S, (const float *)alm, ws, s);
         if (rc) return rc;
         // the subtraction fused into the ring stage's store where it can be
-        rc = launch_ring_resid_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp,
+        rc = launch_ring_resid_f32(p, S, ring_group(S), (const float *)F, (float *)map_tmp,
                                    (const float *)map, s);
         if (rc == GA_OK) {
           subtract = false;
         } else if (rc == GA_ERR_UNSUPPORTED) {
-          rc = launch_ring_synth_f32(p, S, kMfmaGroup, (const float *)F, (float *)map_tmp,
+          rc = launch_ring_synth_f32(p, S, ring_group(S), (const float *)F, (float *)map_tmp,
                                      nullptr, s);
         }
       }
@@ -345,7 +345,7 @@ extern "C" int ga_alm2map_spin_batch_f32(const ga_plan *p, int32_t spin, int32_t
   }
   int rc = launch_spin_synth_f16(p, spin, S, E, B, ws, s);
   if (rc) return rc;
-  return launch_ring_synth_f32(p, 2 * S, kMfmaGroup, (const float *)ws, maps, nullptr, s);
+  return launch_ring_synth_f32(p, 2 * S, ring_group(2 * S), (const float *)ws, maps, nullptr, s);
 }
 
 // ---- graph capture ------------------------------------------------------------

@@ -77,6 +77,20 @@ constexpr int kAutoChunks = -3;
 constexpr int kTwoChunkMinSlabs = 64;
 ga_tuning &tuning();  // the process default
 
+// The FP32 ring buffer of a batch of S shells holds them in groups of
+// G = ring_group(S): complex element (ring, m, s) at
+//   ((ring ceil(S/G) + s/G)(lmax+1) + m) G + s%G.
+// G = 8 for batches of at least 8 shells: each (ring, group, m) row is then
+// 16 real columns, 64 bytes, so the synthesis epilogue's store of a
+// half-wave's 32 columns touches two rows (G = 4: four 32-byte rows, 25 % slower
+// contraction; G = 16: one 128-byte row, but the ring stage, whose workgroups
+// read one shell pair across all m, touches a line per m and is slower by
+// more); smaller batches use G = 4.  Measured at nside 2048, 64 shells:
+// contraction + ring stage 42.4 (G = 4), 38.6 (G = 8), 40.2 ms (G = 16).
+__host__ __device__ inline int ring_group(int nshell) { return nshell >= 8 ? 8 : 4; }
+// log2 of the real columns (2 G) per group
+__host__ __device__ inline int ring_group_cshift(int nshell) { return nshell >= 8 ? 4 : 3; }
+
 // l rows per slab of the FP16-split MFMA Legendre stage (K = 16 per parity)
 constexpr int kSlab = 32;
 __host__ __device__ inline int64_t nslab_of(int64_t lmax, int64_t m) { return (lmax - m + kSlab) / kSlab; }

@@ -58,12 +58,12 @@ int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, co
 
 // Ring stage of synthesis: F -> maps [S][npix] with fused transform.  F holds
 // the shells in groups of G: offset ((ring ngrp + s/G)(lmax+1) + m) G + s%G
-// (G = S is the plain [ring][m][S] layout; the MFMA stage writes G = 8).
+// (G = S is the plain [ring][m][S] layout; the MFMA stages write
+// G = ring_group(S)).
 int launch_ring_synth_f64(const ga_plan *p, int S, int G, const double *F, double *maps,
                           const ga_epilogue *epi, hipStream_t s);
 int launch_ring_synth_f32(const ga_plan *p, int S, int G, const float *F, float *maps,
                           const ga_epilogue *epi, hipStream_t s);
-constexpr int kMfmaGroup = 4;  // shells per group in the MFMA stage's ring buffer
 
 // Ring stage of analysis: maps [S][npix] -> F [ring][m][S] (weighted, phase removed)
 int launch_ring_anal_f64(const ga_plan *p, int S, const double *maps, double *F, hipStream_t s);
@@ -93,7 +93,7 @@ int launch_spin_synth_f32(const ga_plan *p, int spin, int S, const float *E, con
                           float *Fo, hipStream_t s);
 // Spin synthesis Legendre stage on split-FP16 MFMA: E (and B, may be NULL for
 // B = 0) [idx][S] complex64 -> the grouped ring buffer of 2S transforms (Q_s =
-// transform 2s, U_s = 2s + 1; groups of kMfmaGroup) at the start of the FP32
+// transform 2s, U_s = 2s + 1; groups of ring_group(2S)) at the start of the FP32
 // workspace ws of 2S transforms (f32_workspace(p, 2S))
 int launch_spin_synth_f16(const ga_plan *p, int spin, int S, const float *E, const float *B,
                           void *ws, hipStream_t s);

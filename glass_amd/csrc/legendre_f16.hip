@@ -365,25 +365,28 @@ __global__ __launch_bounds__(256) void legendre_f16_start_kernel(
 template <int NT, bool LOAD>
 __device__ __forceinline__ void park_tile(f32x16 (&acc)[2][2][NT], int m, int lmax, int npairs,
                                           int nrings, int ncols, int colbase, int rp0, int lane,
-                                          float *__restrict__ F, float *__restrict__ eqo) {
+                                          float *__restrict__ F, float *__restrict__ eqo,
+                                          int cshift) {
   const int half = lane >> 5, col_in = lane & 31;
   // Buffer addressing from two wave-uniform bases, so that a store costs no
   // address registers: north row rp0 + r at byte r rstride, south row
   // nrings - 1 - rp0 - r at byte (63 - r) rstride of its base; the column
   // part is per lane, the row part per element (scalar).  The host parks in
   // F only where 64 rows span less than 4 GiB.
-  const uint32_t rstride = uint32_t((ncols + 7) / 8) * uint32_t(lmax + 1) * 32u;
+  const uint32_t cg = 1u << cshift;  // real columns per group
+  const uint32_t rstride = uint32_t((ncols + cg - 1) >> cshift) * uint32_t(lmax + 1) * cg * 4u;
   const int r0 = __builtin_amdgcn_readfirstlane(rp0);
-  char *const fnb = reinterpret_cast<char *>(F) + int64_t(r0) * rstride + int64_t(m) * 32;
+  char *const fnb = reinterpret_cast<char *>(F) + int64_t(r0) * rstride + int64_t(m) * cg * 4;
   char *const fsb =
-      reinterpret_cast<char *>(F) + (int64_t(nrings - 1 - r0) - 63) * rstride + int64_t(m) * 32;
+      reinterpret_cast<char *>(F) + (int64_t(nrings - 1 - r0) - 63) * rstride + int64_t(m) * cg * 4;
   const __amdgpu_buffer_rsrc_t rn = __builtin_amdgcn_make_buffer_rsrc(fnb, 0, -1, 0x00020000);
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(fsb, 0, -1, 0x00020000);
 #pragma unroll
   for (int ct = 0; ct < NT; ++ct) {
     const int col = colbase + ct * 32 + col_in;
     const bool colok = col < ncols;
-    const uint32_t cofs = uint32_t(col >> 3) * uint32_t(lmax + 1) * 32u + uint32_t(col & 7) * 4u;
+    const uint32_t cofs =
+        uint32_t(col >> cshift) * uint32_t(lmax + 1) * cg * 4u + uint32_t(col & (cg - 1)) * 4u;
     const uint32_t vn = cofs + uint32_t(4 * half) * rstride;
     const uint32_t vs = cofs + uint32_t(4 * (1 - half)) * rstride;
 #pragma unroll
@@ -474,6 +477,7 @@ struct f16_geom {
   const double2 *st;
   const int32_t *stk;
   int ngroups;           // groups of 64 ring pairs
+  int cshift;            // ring buffer: 1 << cshift real columns per group
   int sep;
   int lmax, npairs, nrings, ncols, nct, nrg, ncg, lam_exp, flush, nchunk, nt, blk_begin;
   int mgroup;     // consecutive m per XCD turn (1 or 4)
@@ -495,9 +499,9 @@ __device__ __forceinline__ float pair_swap(float v) {
 // Unscale by 2^-(8 + shift of the column), north = E + O, south = E - O,
 // times the ring phase e^{i m phi0} (each lane computes its own ring's phase
 // in FP64; rows fetch it by shuffle; the (re, im) columns of a shell sit in
-// lanes (2c, 2c + 1), exchanged by DPP).  Ring buffer layout: real column c
-// of ring r at
-//   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8).
+// lanes (2c, 2c + 1), exchanged by DPP).  Ring buffer layout (cg = 1 << cshift
+// real columns per group, ring_group): real column c of ring r at
+//   ((r ngrp + c/cg)(lmax+1) + m) cg + c%cg,  ngrp = ceil(ncols / cg).
 // Addresses: per element a wave-uniform ring base (the row's ring for the
 // lower half-wave) plus a per-lane 32-bit offset fixed for the whole tile
 // (the column part, and 4 rings for the other half-wave), so that a store
@@ -508,7 +512,7 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
                                            int rpc, int lane, const int32_t *__restrict__ ring_pnum,
                                            const int32_t *__restrict__ ring_pden,
                                            const int32_t *__restrict__ colexp,
-                                           float *__restrict__ F, int lam_exp) {
+                                           float *__restrict__ F, int lam_exp, int cshift) {
   const int half = lane >> 5, col_in = lane & 31;
   float phc, phs;
   {
@@ -524,7 +528,8 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
   const bool is_im = (col_in & 1) != 0;
   const float zf = (m == 0 && is_im) ? 0.0f : 1.0f;
   const float sg = is_im ? zf : -zf;
-  const int64_t rstride = int64_t((ncols + 7) / 8) * (lmax + 1) * 8;  // floats per ring
+  const int cg = 1 << cshift;  // real columns per group
+  const int64_t rstride = int64_t((ncols + cg - 1) >> cshift) * (lmax + 1) * cg;  // floats per ring
   float fac[NT];
   bool cok[NT];
   uint32_t noff[NT], soff[NT];
@@ -534,7 +539,7 @@ __device__ __forceinline__ void store_tile(const f32x16 (&acc)[2][2][NT], int m,
     cok[ct] = col < ncols;
     fac[ct] = cok[ct] ? ldexpf(1.0f, -(lam_exp + colexp[col])) : 0.0f;
     // byte offsets (< 2^32: the column part of a ring plus 4 rings)
-    const uint32_t cofs = uint32_t(((col >> 3) * (lmax + 1) + m) * 8 + (col & 7)) * 4u;
+    const uint32_t cofs = uint32_t((((col >> cshift) * (lmax + 1) + m) << cshift) + (col & (cg - 1))) * 4u;
     noff[ct] = cofs + uint32_t(half * 4 * rstride) * 4u;
     soff[ct] = cofs + uint32_t((1 - half) * 4 * rstride) * 4u;
   }
@@ -727,7 +732,7 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   run_slabs(two ? s + (nsl - s) / 2 : nsl);
   if (two) {
     if (SEP) park_lanes<NT, false>(acc, g.part + park_slot(m, rem, G, wave, NT) + lane);
-    else park_tile<NT, false>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
+    else park_tile<NT, false>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo, g.cshift);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -736,7 +741,7 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
         for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] = f32x16{};
     run_slabs(nsl);
     if (SEP) park_lanes<NT, true>(acc, g.part + park_slot(m, rem, G, wave, NT) + lane);
-    else park_tile<NT, true>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo);
+    else park_tile<NT, true>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, lane, F, eqo, g.cshift);
   }
   if (FLUSH) {
 #pragma unroll
@@ -747,7 +752,7 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
         for (int ct = 0; ct < NT; ++ct) acc[p][t][ct] += mst[p][t][FLUSH ? ct : 0];
   }
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
-                 colexp, F, lam_exp);
+                 colexp, F, lam_exp, g.cshift);
 }
 
 template <int NT, int MINW, bool FLUSH, bool SEP>
@@ -912,7 +917,7 @@ __global__ __launch_bounds__(512, 1) void legendre_synth_f16_pc_kernel(
   }
   if (!active) return;
   store_tile<NT>(acc, m, lmax, npairs, nrings, ncols, colbase, rp0, rpc, lane, ring_pnum, ring_pden,
-                 colexp, F, lam_exp);
+                 colexp, F, lam_exp, ring_group_cshift(ncols / 2));
 }
 
 }  // namespace
@@ -963,6 +968,7 @@ static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
   g.st = p->d_f16_st;
   g.stk = p->d_f16_k;
   g.ngroups = p->f16_ngroups;
+  g.cshift = ring_group_cshift(S);
   g.ring_pnum = p->d_ring_pnum;
   g.ring_pden = p->d_ring_pden;
   g.slab_off = p->d_slab_off;

@@ -207,8 +207,9 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_mfma_kernel(
   // the phase of its own ring (FP64, exact argument reduction); rows fetch it
   // by shuffle, and the (re, im) columns of a shell sit in lanes (2c, 2c+1).
   // Ring buffer layout: real column c (shell c/2) of ring r lives at
-  //   ((r ngrp + c/8)(lmax+1) + m) 8 + c%8,  ngrp = ceil(ncols / 8),
-  // i.e. shells in groups of 4, each (ring, group, m) row 32 contiguous bytes.
+  //   ((r ngrp + c/gc)(lmax+1) + m) gc + c%gc,  ngrp = ceil(ncols / gc),
+  // i.e. shells in groups of ring_group(S) = gc / 2, each (ring, group, m) row
+  // 4 gc contiguous bytes.
   float phc, phs;
   {
     const int pnum = ring_pnum[rpc], pden = ring_pden[rpc];
@@ -217,7 +218,8 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_mfma_kernel(
     phc = float(pc);
     phs = float(ps);
   }
-  const int ngrp = (ncols + 7) / 8;
+  const int cshift = ring_group_cshift(ncols / 2), gc = 1 << cshift;
+  const int ngrp = (ncols + gc - 1) >> cshift;
   const bool is_im = (col_in & 1) != 0;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -238,10 +240,10 @@ __global__ __launch_bounds__(256, MINW) void legendre_synth_mfma_kernel(
         if (rpo < npairs && col < ncols) {
           const float nrz = (m == 0 && is_im) ? 0.0f : nrot;
           const float srz = (m == 0 && is_im) ? 0.0f : srot;
-          const int64_t gofs = (int64_t(col >> 3) * (lmax + 1) + m) * 8 + (col & 7);
-          F[int64_t(rpo) * ngrp * (lmax + 1) * 8 + gofs] = nrz;
+          const int64_t gofs = ((int64_t(col >> cshift) * (lmax + 1) + m) << cshift) + (col & (gc - 1));
+          F[int64_t(rpo) * ngrp * (lmax + 1) * gc + gofs] = nrz;
           if (rpo < npairs - 1)
-            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * 8 + gofs] = srz;
+            F[int64_t(nrings - 1 - rpo) * ngrp * (lmax + 1) * gc + gofs] = srz;
         }
       }
     }

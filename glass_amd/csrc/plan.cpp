@@ -577,8 +577,10 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch) {
 f32_ws_layout f32_workspace(const ga_plan *p, int nbatch, const ga_tuning &tu) {
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   f32_ws_layout w;
-  // the ring buffer holds whole groups of 8 shells (the MFMA stage layout)
-  const size_t nb = size_t(nbatch + 7) / 8 * 8;
+  // the ring buffer holds whole groups of ring_group(nbatch) shells (and at
+  // least a multiple of 8)
+  const size_t gq = size_t(std::max(8, ring_group(nbatch)));
+  const size_t nb = (size_t(nbatch) + gq - 1) / gq * gq;
   w.f_bytes = up(size_t(p->nrings) * size_t(p->lmax + 1) * nb * 8);
   const size_t nct = size_t(2 * nbatch + 31) / 32;  // 32-column tiles
   // per (slab, parity, tile): 64 lanes x (8 high + 8 low halves) = 2 KiB
@@ -609,7 +611,8 @@ f32_ws_layout f32_workspace(const ga_plan *p, int nbatch, const ga_tuning &tu) {
 }
 
 bool f16_park_fits(const ga_plan *p, int nbatch) {
-  const uint64_t rstride = uint64_t((2 * nbatch + 7) / 8) * uint64_t(p->lmax + 1) * 32;
+  const uint64_t cg = uint64_t(2 * ring_group(nbatch));  // real columns per group
+  const uint64_t rstride = uint64_t((2 * nbatch + cg - 1) / cg) * uint64_t(p->lmax + 1) * cg * 4;
   return 64 * rstride < (uint64_t(1) << 32);
 }
 

@@ -241,8 +241,10 @@ __global__ __launch_bounds__(256, MINW) void spin_synth_f16_kernel(
     const int col = colbase + ct * 32 + col_in;
     fac[ct] = col < ncols ? ldexpf(1.0f, -(kLamExp + colexp[col])) : 0.0f;
   }
-  const int ngrp = (2 * ncols + 7) / 8;  // 2S transforms = 4S real columns, 8 per group
-  const int64_t ring_stride = int64_t(ngrp) * (lmax + 1) * 8;
+  // 2S transforms = 4S real columns, in groups of ring_group(2S)
+  const int cshift = ring_group_cshift(ncols), gc = 1 << cshift;
+  const int ngrp = (2 * ncols + gc - 1) >> cshift;
+  const int64_t ring_stride = int64_t(ngrp) * (lmax + 1) * gc;
   const bool is_im = (col_in & 1) != 0;
   const float ss = (spin & 1) ? -1.0f : 1.0f;
 #pragma unroll
@@ -273,8 +275,8 @@ __global__ __launch_bounds__(256, MINW) void spin_synth_f16_kernel(
       }
       if (rpo < npairs && col < ncols) {
         const int qc = 4 * (col >> 1) + (col & 1), uc = qc + 2;
-        const int64_t oq = (int64_t(qc >> 3) * (lmax + 1) + m) * 8 + (qc & 7);
-        const int64_t ou = (int64_t(uc >> 3) * (lmax + 1) + m) * 8 + (uc & 7);
+        const int64_t oq = ((int64_t(qc >> cshift) * (lmax + 1) + m) << cshift) + (qc & (gc - 1));
+        const int64_t ou = ((int64_t(uc >> cshift) * (lmax + 1) + m) << cshift) + (uc & (gc - 1));
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           if (h == 1 && rpo >= npairs - 1) break;  // the equator ring has no mirror

@@ -162,8 +162,9 @@ int ga_alm2map_batch_f32_noalloc(const ga_plan *plan, int32_t nshell, const floa
 /* The two stages of ga_alm2map_batch_f32, callable separately (for timing
  * and for callers that keep the ring buffer): the MFMA Legendre stage
  * alm [idx][S] -> ring buffer F, which occupies the start of the workspace
- * (ga_workspace_size(plan, S, 4) bytes; shells in groups of 4, complex
- * element (ring, m, s) at ((ring ceil(S/4) + s/4)(lmax+1) + m) 4 + s%4), and
+ * (ga_workspace_size(plan, S, 4) bytes; shells in groups of G = 8 when
+ * S >= 8, else G = 4, complex element (ring, m, s) at
+ * ((ring ceil(S/G) + s/G)(lmax+1) + m) G + s%G), and
  * the ring-FFT stage F (= the workspace pointer) -> maps [S][npix] with the
  * fused transform. */
 int ga_legendre_synth_batch_f32(const ga_plan *plan, int32_t nshell, const float *alm,
