@@ -53,8 +53,10 @@ def test_alm2map_ignores_imag_m0_and_torch_roundtrip():
     assert relerr(got.cpu().numpy(), ref) < 1e-12
 
 
+# S = 1, 3, 5: ring buffer in groups of 4 shells; S = 9, 16, 17, 40: groups of 8
+# (9 and 17: a partly filled last group)
 @pytest.mark.parametrize("nside,lmax,S", [(4, 11, 1), (8, 23, 3), (16, 40, 16), (32, 63, 17),
-                                          (64, 127, 40), (128, 256, 5)])
+                                          (32, 70, 9), (64, 127, 40), (128, 256, 5)])
 def test_alm2map_fp32_mfma(nside, lmax, S):
     print(f"alm2map fp32 nside={nside} lmax={lmax} S={S}", flush=True)
     rng = np.random.default_rng(nside + S)
