@@ -87,7 +87,8 @@ enum ga_tune_key {
                              automatic = 4), so that with 4 the four m of a 128-byte
                              line of the ring buffer are written by one XCD's L2
                              (bit-identical results; 37.4 -> 33.9 ms at nside 2048,
-                             64 shells) */,
+                             64 shells with the ring buffer in groups of 4 shells;
+                             no measurable difference with groups of 8) */,
   GA_TUNE_CHUNK_MIN = 9   /* two-chunk accumulation (GA_TUNE_F16_FLUSH -2 / -3): the
                              least l-range, in slabs of 32 rows, that is split (1 ..
                              4096; 0 = automatic = 64: shorter chains add little to
