@@ -68,6 +68,29 @@ def test_alm2map_fp32_mfma(nside, lmax, S):
         assert e < 1e-5, (s, e)
 
 
+@pytest.mark.parametrize("path", [1, 2])
+def test_alm2map_fp32_both_legendre_paths_groups_of_8(path):
+    """The FP32 MFMA path and the split-FP16 path on a batch of 12 shells,
+    which the ring buffer holds in groups of 8 (the second partly filled)."""
+    from glass_amd import _lib
+
+    nside, lmax, S = 32, 70, 12
+    print(f"alm2map fp32 path={path} nside={nside} lmax={lmax} S={S}", flush=True)
+    rng = np.random.default_rng(path + 100)
+    alms = [rand_alm(lmax, rng, lambda l: 1e-4 / (l + 1.0) ** 1.2) for _ in range(S)]
+    plan = _lib.get_plan(nside, lmax)
+    old = plan.get_tuning(_lib.TUNE_MFMA_PATH)
+    plan.set_tuning(_lib.TUNE_MFMA_PATH, path)
+    try:
+        got = hp.alm2map(alms, nside, pol=False, precision="fp32")
+    finally:
+        plan.set_tuning(_lib.TUNE_MFMA_PATH, old)
+    for s in range(S):
+        ref = O.alm2map(alms[s], nside, nthreads=4)
+        e = relerr(np.asarray(got[s], dtype=np.float64), ref)
+        assert e < 1e-5, (s, e)
+
+
 @pytest.mark.parametrize("nside", [64, 4096])
 def test_alm2map_fp32_mfma_lmax8192(nside):
     """lmax 8192 (the nside 4096 serving requests): the split-FP16 synthesis
