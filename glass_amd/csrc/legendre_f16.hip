@@ -956,6 +956,14 @@ int launch_legendre_prepare_f16(const ga_plan *p, int S, const float *alm, void 
 
 // the f16_geom of plan p for S shells (flush resolved, nt 1 or 2) and its
 // block count
+// whether the epilogue's 32-bit per-lane byte offsets cover the ring buffer
+// rows of S shells: a ring's columns plus 4 rings within 4 GiB
+static bool f16_offsets_fit(const ga_plan *p, int S) {
+  const uint64_t cg = uint64_t(2 * ring_group(S));
+  const uint64_t ring_bytes = (uint64_t(2 * S) + cg - 1) / cg * cg * uint64_t(p->lmax + 1) * 4;
+  return 5 * ring_bytes < (uint64_t(1) << 32);
+}
+
 static f16_geom f16_geometry(const ga_plan *p, int S, float *F, const void *bws,
                              const int32_t *colexp, int nt, int flush, void *park,
                              int mgroup, int chunk_min, int64_t *nblk) {
@@ -1026,6 +1034,8 @@ int launch_legendre_contract_f16(const ga_plan *p, int S, float *F, const void *
   GA_REQUIRE(flush != -2 || (park && f16_park_fits(p, S)), GA_ERR_UNSUPPORTED,
              "two-chunk accumulation needs 64 ring rows of %d shells within 4 GiB; "
              "use one chain (-1) or register master sums (n > 0)", S);
+  GA_REQUIRE(f16_offsets_fit(p, S), GA_ERR_UNSUPPORTED,
+             "%d shells at lmax %d exceed the synthesis epilogue's 32-bit offsets", S, p->lmax);
   if (flush > 0) nt = 1;
   if (nt != 1 && nt != 2 && nt != 4) nt = ncols <= 32 ? 1 : 2;
   if (nt == 4) {
@@ -1083,6 +1093,9 @@ int launch_legendre_contract_grouped(int ngroup, const ga_plan *const *plans, co
     int nt = tu.mfma_nt;
     if (nt != 1 && nt != 2) nt = 2 * S[i] <= 32 ? 1 : 2;
     int64_t nblk = 0;
+    GA_REQUIRE(f16_offsets_fit(p, S[i]), GA_ERR_UNSUPPORTED,
+               "geometry %d: %d shells at lmax %d exceed the epilogue's 32-bit offsets", i, S[i],
+               p->lmax);
     grp.g[i] = f16_geometry(p, S[i], reinterpret_cast<float *>(base), base + w.b_off,
                             reinterpret_cast<const int32_t *>(base + w.e_off), nt, acc_form,
                             base + w.p_off, tu.m_group, tu.chunk_min, &nblk);
