@@ -105,7 +105,10 @@ const char *ga_version(void);
 
 /* HEALPix geometry and Legendre/FFT tables for one (nside, lmax) on the current
  * device.  Allocates the plan's own read-only tables; no allocation happens in
- * any other entry point.  Replaces the per-call set-up inside healpy.alm2map /
+ * any other entry point.  One of the tables (where each wave of the split-FP16
+ * synthesis starts its slab loop, and its recurrence state there) is computed
+ * by a kernel on the null stream, which this call waits for: create plans
+ * outside stream capture.  Replaces the per-call set-up inside healpy.alm2map /
  * healpy.map2alm (glass/healpix.py:71, :270). */
 int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out);
 int ga_plan_destroy(ga_plan *plan);
