@@ -68,6 +68,22 @@ def test_alm2map_fp32_mfma(nside, lmax, S):
         assert e < 1e-5, (s, e)
 
 
+@pytest.mark.parametrize("nside,S", [(301, 3), (301, 10), (300, 9)])
+def test_alm2map_fp32_non_power_of_two_nside(nside, S):
+    """RING-scheme nside that is not a power of two: the equatorial rings
+    (r = nside, odd for 301) are wide Bluestein rings, folded in (k1, r - k1)
+    pairs; 3 shells are held in groups of 4, 9 and 10 in groups of 8."""
+    lmax = 2 * nside - 2
+    print(f"alm2map fp32 nside={nside} lmax={lmax} S={S}", flush=True)
+    rng = np.random.default_rng(nside * 10 + S)
+    alms = [rand_alm(lmax, rng, lambda l: 1e-4 / (l + 1.0) ** 1.2) for _ in range(S)]
+    got = hp.alm2map(alms, nside, pol=False, precision="fp32")
+    for s in (0, S // 2, S - 1):
+        ref = O.alm2map(alms[s], nside, nthreads=4)
+        e = relerr(np.asarray(got[s], dtype=np.float64), ref)
+        assert e < 1e-5, (s, e)
+
+
 @pytest.mark.parametrize("path", [1, 2])
 def test_alm2map_fp32_both_legendre_paths_groups_of_8(path):
     """The FP32 MFMA path and the split-FP16 path on a batch of 12 shells,
