@@ -673,8 +673,19 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
   // buffer; every step reads its pair with a broadcast LDS load.  Only this
   // wave touches its buffers, so no workgroup barrier is needed.
   __shared__ __align__(16) double cbuf[4][2][kSlab];
-  const int lastc = lmax - m;  // last valid row index of this m
-  double cpre = co[min(kSlab * s + (lane & (kSlab - 1)), lastc)];
+  // Coefficient and B-operand loads: a wave-uniform base plus a 32-bit lane
+  // offset fixed for the whole loop (no per-slab address arithmetic).  The
+  // coefficients are not clamped at the end of m: the plan pads gam, and
+  // rows past this m's last one only feed steps whose values are discarded.
+  // (buffer loads: scalar base and offset per slab, the lane's offset in a
+  // VGPR; this m's B operands span less than 4 GiB)
+  const uint32_t coff = uint32_t(lane & (kSlab - 1)) * 8u, boff = uint32_t(lane) * 16u;
+  const __amdgpu_buffer_rsrc_t crs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(co), 0, -1, 0x00020000);
+  const __amdgpu_buffer_rsrc_t brs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint4 *>(B + s0 * 2 * nct * 128), 0, -1, 0x00020000);
+  double cpre = __builtin_bit_cast(
+      double, __builtin_amdgcn_raw_buffer_load_b64(crs, coff, kSlab * 8 * s, 0));
   if (lane < kSlab) cbuf[wave][s & 1][lane] = cpre;
   __builtin_amdgcn_wave_barrier();
 
@@ -689,12 +700,14 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
       for (int p = 0; p < 2; ++p)
   #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
-          const uint4 *__restrict__ u = B + (((s0 + s) * 2 + p) * nct + ctile[ct]) * 128;
-          bh[p][ct] = u[lane];
-          bl[p][ct] = u[64 + lane];
+          const int so = ((s * 2 + p) * nct + ctile[ct]) * 2048;
+          bh[p][ct] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(brs, boff, so, 0));
+          bl[p][ct] =
+              __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(brs, boff + 1024, so, 0));
         }
-      // next slab's coefficients (clamped past the end; never used there)
-      cpre = co[min(kSlab * (s + 1) + (lane & (kSlab - 1)), lastc)];
+      // next slab's coefficients (past the end of m: never used)
+      cpre = __builtin_bit_cast(
+          double, __builtin_amdgcn_raw_buffer_load_b64(crs, coff, kSlab * 8 * (s + 1), 0));
       const double *__restrict__ cb = cbuf[wave][cur];
 
       uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)

@@ -349,7 +349,12 @@ extern "C" int ga_plan_create(int32_t nside, int32_t lmax, ga_plan **out) {
     if ((rc = upload(&p->d_ab, ab.data(), ab.size() * sizeof(double2), &p->table_bytes))) return fail(rc);
     // normalised form for the split-FP16 synthesis: y_l = lambda_l / c_l
     {
-      std::vector<double> gam(size_t(p->nalm + 4), 0.0), csc(size_t(p->nalm + 4), 1.0);
+      // gam is padded by 64 entries: the synthesis loads a whole slab of 32
+      // coefficients (and the next slab's) without clamping at the end of m;
+      // entries past an m's last row belong to the next m (or the padding)
+      // and only feed recurrence steps after that m's last row, whose values
+      // are discarded
+      std::vector<double> gam(size_t(p->nalm + 64), 0.0), csc(size_t(p->nalm + 4), 1.0);
       double bound = 0.0;
       for (int64_t m = 0; m <= lmax; ++m) {
         const int64_t b0 = hp_index(lmax, m, m);
