@@ -694,6 +694,10 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
     for (; s < end; ++s) {
       const int l0 = m + kSlab * s;
       const int cur = s & 1;
+      // next slab's coefficients (past the end of m: never used), issued
+      // first: they are written to LDS before the MFMAs need the B operands
+      cpre = __builtin_bit_cast(
+          double, __builtin_amdgcn_raw_buffer_load_b64(crs, coff, kSlab * 8 * (s + 1), 0));
       // B operands of this slab, consumed after the recurrence below
       uint4 bh[2][NT], bl[2][NT];
   #pragma unroll
@@ -705,9 +709,6 @@ __device__ __forceinline__ void contract_blocks(const f16_geom &g, const int b) 
           bl[p][ct] =
               __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(brs, boff + 1024, so, 0));
         }
-      // next slab's coefficients (past the end of m: never used)
-      cpre = __builtin_bit_cast(
-          double, __builtin_amdgcn_raw_buffer_load_b64(crs, coff, kSlab * 8 * (s + 1), 0));
       const double *__restrict__ cb = cbuf[wave][cur];
 
       uint32_t ph[2][8], pl[2][8];  // [parity][group]: packed (k = 2g, 2g + 1)
