@@ -179,9 +179,9 @@ def test_device_rng_statistics_and_batch_independence():
     assert abs(np.mean(samples) / var - 1) < 0.15
 
 
-def test_config1_cpu_case_parity():
-    """configs[0] restated: 10 shells, nside 128, lmax 256, ncorr 1, seed 0,
-    Lognormal(1); FP64 path vs the oracle (and the FP32 path within 1e-5)."""
+def test_ten_lognormal_shells_nside128_parity():
+    """The reference's CPU-sized case: 10 shells, nside 128, lmax 256, ncorr 1,
+    seed 0, Lognormal(1); FP64 path vs the oracle (and the FP32 path within 1e-5)."""
     lmax, nside, n = 256, 128, 10
     gls = _gls(n, lmax, 1, rho=(0.5,))
     fields = [glass_amd.grf.Lognormal(1.0) for _ in range(n)]
@@ -195,7 +195,7 @@ def test_config1_cpu_case_parity():
         rms = np.sqrt(np.mean(r**2))
         e64.append(np.max(np.abs(g - r)) / rms)
         e32.append(np.max(np.abs(g32 - r)) / rms)
-    print(f"C1 worst of {n} shells vs oracle: fp64 {max(e64):.2e}, fp32 {max(e32):.2e}",
+    print(f"10 shells, nside 128: worst of {n} shells vs oracle: fp64 {max(e64):.2e}, fp32 {max(e32):.2e}",
           flush=True)
     assert max(e64) < 1e-12 and max(e32) < 1e-5
 

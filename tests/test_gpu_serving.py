@@ -84,7 +84,7 @@ def test_c5_geometries_alone_mixed_and_fp64():
     assert sorted(c.rid for c in sched.completed) == list(range(len(reqs)))
     worst = {}
     for r in reqs:
-        print(f"C5 request nside={r.nside} lmax={r.lmax} shells={r.nshell}", flush=True)
+        print(f"serving request nside={r.nside} lmax={r.lmax} shells={r.nshell}", flush=True)
         mixed = ex.results[r.rid]
         eng = LognormalShells(r.spectra(), r.nside, ncorr=r.ncorr, shift=r.shift, seed=r.seed,
                               lmax=r.lmax)

@@ -217,7 +217,7 @@ def test_almxfl():
     np.testing.assert_allclose(a, O.almxfl(alm, fl), rtol=1e-15, atol=0)
 
 
-def test_config2_single_map_fp64_round_trip():
+def test_single_map_nside1024_fp64_round_trip():
     """Single-map drop-in SHT, plain FP64 path, at its full size: complex128
     alm from seed 1 with C_l = (l+1)^-2 (a_lm = sqrt(C_l / 2) (z1 + i z2), real
     at m = 0), alm2map to nside 1024 (12.58M pixels), map2alm back (healpy's 3
@@ -237,11 +237,11 @@ def test_config2_single_map_fp64_round_trip():
     mask = ref != 0
     got = m.cpu().numpy()
     err = np.max(np.abs(got[mask] - ref[mask])) / np.sqrt(np.mean(ref[mask] ** 2))
-    print(f"config2 synthesis vs oracle ({mask.sum()} pixels): {err:.2e}", flush=True)
+    print(f"nside 1024 synthesis vs oracle ({mask.sum()} pixels): {err:.2e}", flush=True)
     assert err < 1e-11
     rt = {}
     for it in (0, 3):
         back = hp.map2alm(m, lmax=lmax, iter=it).cpu().numpy()
         rt[it] = float(np.sqrt(np.mean(np.abs(back - alm) ** 2) / np.mean(np.abs(alm) ** 2)))
-    print(f"config2 round trip rms rel error: iter 0 {rt[0]:.3e}, iter 3 {rt[3]:.3e}", flush=True)
+    print(f"nside 1024 round trip rms rel error: iter 0 {rt[0]:.3e}, iter 3 {rt[3]:.3e}", flush=True)
     assert rt[3] < rt[0] and rt[3] < 5e-2

@@ -132,5 +132,5 @@ def test_c4_geometries_graph_eager_and_fp64():
             alm = hp.almxfl(alm, fl)
             q, u = hp.alm2map_spin([alm, torch.zeros_like(alm)], nside, 2, lmax)
             e_q, e_u = rel(eng.gamma[j, 0], q), rel(eng.gamma[j, 1], u)
-            print(f"C4 nside={nside} map {j}: {e_map:.2e} gamma {e_q:.2e} {e_u:.2e}", flush=True)
+            print(f"uneven batch nside={nside} map {j}: {e_map:.2e} gamma {e_q:.2e} {e_u:.2e}", flush=True)
             assert e_map < 1e-5 and e_q < 1e-5 and e_u < 1e-5, (nside, j, e_map, e_q, e_u)
