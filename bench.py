@@ -293,7 +293,7 @@ def accuracy_all_shells(eng, S: int):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(shells and requests are independent, so N GPUs run N independent batches).",
               file=sys.stderr)
         sys.exit(2)

@@ -205,7 +205,7 @@ def map2alm(maps, *, lmax: int | None = None, pol: bool = True,
     T = _lib.torch()
     single = getattr(maps, "ndim", None) == 1
     if not single and pol and len(_as_list(maps)) == 3:
-        raise NotImplementedError("polarised (TQU) analysis is not part of this build")
+        raise NotImplementedError("polarised (TQU) analysis is not implemented")
     lst = [maps] if single else _as_list(maps)
     nside = npix2nside(lst[0].shape[-1])
     if lmax is None:

@@ -8,7 +8,7 @@ alm2map / alm2map_spin.  The factors are the reference's:
 so kappa -> gamma is -sqrt((l-1)(l+2) / (l(l+1))) for l >= 2 and 0 below.
 With ``discretized=True`` (the default, as in the reference) the factor is
 multiplied by pw2 / pw0 of ``healpix.pixwin(nside, lmax=lmax, pol=True)``,
-which this build computes on the GPU from the pixel-window definition; an
+which glass_amd computes on the GPU from the pixel-window definition; an
 explicit ratio can be passed as ``pixwin_ratio`` instead.
 """
 

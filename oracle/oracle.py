@@ -1,7 +1,7 @@
 """FP64 CPU oracle for the GLASS field-generation path.
 
 TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, by ``__graft_entry__.smoke``
-and by the CPU-baseline leg of ``bench.py`` -- always as the checker, never as
+and by the CPU-baseline leg of ``bench.py`` -- always to check results, never as
 the thing measured or shipped.  The product package ``glass_amd`` never imports
 this module.
 
@@ -41,8 +41,8 @@ Pinning (what checks each function, all in ``tests/test_oracle.py``):
                     same seed -> same first field for ncorr None vs 1; two
                     identical gls -> field 1 equals field 0.
 * philox4x32_10  -- the published known-answer vectors of Philox4x32-10
-                    (Salmon et al., SC'11, Random123 distribution).  The device
-                    generator of this build (not the reference's numpy stream)
+                    (Salmon et al., SC'11, Random123 distribution).  The glass_amd
+                    device generator (not the reference's numpy stream)
                     is then checked against device_normals built on it.
 
 * MultiPlaneConvergence -- the reference's own identities
