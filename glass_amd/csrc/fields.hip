@@ -31,20 +31,29 @@ __device__ __forceinline__ int glass_l(int64_t g) {
   return l;
 }
 
-// standard complex normal pair (re, im) for GLASS-order index g of shell t:
-// Box-Muller on two Philox words, u1 = (w0 + 1/2) 2^-32 in (0, 1] (tail to
-// sqrt(-2 ln 2^-33) = 6.76 sigma) and u2 = (w1 >> 8) 2^-24 in [0, 1), with the
-// hardware log2 / sqrt / sin(2 pi x) / cos(2 pi x) in FP32 (relative error a
-// few 1e-7: far below what the statistics of the field can resolve), widened
-// to FP64 so that every path (FP32 batch, FP64 reference) sees the same values
-__device__ __forceinline__ double2 device_normal_pair(uint64_t seed, int64_t t, int64_t g) {
-  u32x4 c{{uint32_t(g), uint32_t(uint64_t(g) >> 32), uint32_t(t), uint32_t(uint64_t(t) >> 32)}};
-  const u32x4 r = philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
-  const float u1 = (float(r.v[0]) + 0.5f) * 0x1p-32f;
-  const float u2 = float(r.v[1] >> 8) * 0x1p-24f;
+// Standard complex normal pairs (re, im) for GLASS-order index g: one
+// Philox4x32-10 call with counter (g lo, g hi, u lo, u hi) serves the shells
+// t = 2u (words 0, 1) and t = 2u + 1 (words 2, 3).  Box-Muller on a word pair
+// (w0, w1): u1 = (w0 + 1/2) 2^-32 in (0, 1] (tail to sqrt(-2 ln 2^-33) =
+// 6.76 sigma) and u2 = (w1 >> 8) 2^-24 in [0, 1), with the hardware log2 /
+// sqrt / sin(2 pi x) / cos(2 pi x) in FP32 (relative error a few 1e-7: far
+// below what the statistics of the field can resolve), widened to FP64 so
+// that every path (FP32 batch, FP64 reference) sees the same values.
+__device__ __forceinline__ u32x4 device_normal_words(uint64_t seed, int64_t u, int64_t g) {
+  u32x4 c{{uint32_t(g), uint32_t(uint64_t(g) >> 32), uint32_t(u), uint32_t(uint64_t(u) >> 32)}};
+  return philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+}
+__device__ __forceinline__ double2 normal_from_words(uint32_t w0, uint32_t w1) {
+  const float u1 = (float(w0) + 0.5f) * 0x1p-32f;
+  const float u2 = float(w1 >> 8) * 0x1p-24f;
   const float rad = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(u1));
   const float sn = __builtin_amdgcn_sinf(u2), cs = __builtin_amdgcn_cosf(u2);
   return double2{double(rad * cs), double(rad * sn)};
+}
+// the pair of shell t (t >= 0)
+__device__ __forceinline__ double2 device_normal_pair(uint64_t seed, int64_t t, int64_t g) {
+  const u32x4 r = device_normal_words(seed, t >> 1, g);
+  return (t & 1) ? normal_from_words(r.v[2], r.v[3]) : normal_from_words(r.v[0], r.v[1]);
 }
 
 // State per mode l (capacity kmax): M [kmax][kmax] row-major, a [kmax], s,
@@ -174,12 +183,22 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
   int l = 0, m = 0;
   if (valid) hp_decode(i, nl - 1, l, m);
   const int64_t g = int64_t(l) * (l + 1) / 2 + m;
-  // window zz[j] = z_{t - K + j} for the current shell t
+  // window zz[j] = z_{t - K + j} for the current shell t; (sw0, sw1): the
+  // Philox words of shell t + 1 when t is even (one call serves both)
   double2 zz[K + 1];
 #pragma unroll
-  for (int j = 0; j <= K; ++j) {
+  for (int j = 0; j < K; ++j) {
     const int64_t tz = int64_t(first_shell) - K + j;
     zz[j] = (tz >= 0 && valid) ? device_normal_pair(seed, tz, g) : double2{0.0, 0.0};
+  }
+  uint32_t sw0, sw1;
+  {
+    const u32x4 r = device_normal_words(seed, int64_t(first_shell) >> 1, g);
+    const bool odd = first_shell & 1;
+    zz[K] = valid ? normal_from_words(odd ? r.v[2] : r.v[0], odd ? r.v[3] : r.v[1])
+                  : double2{0.0, 0.0};
+    sw0 = r.v[2];
+    sw1 = r.v[3];
   }
   // weights of shell s for this l: wrow[s * wstride + j], j = 0..K
   const double *wrow = w + int64_t(l) * (K + 1);
@@ -200,7 +219,17 @@ __global__ __launch_bounds__(kFaBlock) void fields_alm_window_kernel(
       if (s > 0) {
 #pragma unroll
         for (int j = 0; j < K; ++j) zz[j] = zz[j + 1];
-        zz[K] = valid ? device_normal_pair(seed, t, g) : double2{0.0, 0.0};
+        // t odd: shell t - 1 (the previous one) was even and left its
+        // partner's words; t even: a new call (block-uniform branch)
+        uint32_t w0 = sw0, w1 = sw1;
+        if (!(t & 1)) {
+          const u32x4 r = device_normal_words(seed, t >> 1, g);
+          w0 = r.v[0];
+          w1 = r.v[1];
+          sw0 = r.v[2];
+          sw1 = r.v[3];
+        }
+        zz[K] = valid ? normal_from_words(w0, w1) : double2{0.0, 0.0};
       }
       double re = 0.0, im = 0.0;
       if (t >= K) {

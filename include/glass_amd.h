@@ -323,7 +323,9 @@ int ga_iternorm_batch(int32_t nl, int32_t kprev, int32_t k, int32_t kmax, int32_
  *   - host-drawn (numpy PCG64 stream): zring = device ring buffer of nslot
  *     GLASS-order complex128 vectors, z_t in slot t % nslot (nslot >=
  *     nshell + k so that every shell of the call sees its window); or
- *   - counter-based Philox4x32-10 (seed, shell, index) when zring == NULL.
+ *   - counter-based Philox4x32-10 (seed, shell, index) when zring == NULL:
+ *     key = seed, counter = (index, shell / 2), words 0-1 of the call for
+ *     an even shell and 2-3 for an odd one, Box-Muller on each pair.
  * Output: batch layout [idx][ldo] complex64 (out_f32; shell s of the call in
  * column s, ldo >= nshell, 0 = nshell, so that several calls can fill the
  * column ranges of one shared batch) or, for nshell == 1, healpy-order

@@ -475,16 +475,20 @@ def philox4x32_10(ctr: np.ndarray, key: np.ndarray) -> np.ndarray:
 
 def device_normals(seed: int, shell: int, g: np.ndarray) -> np.ndarray:
     """Complex standard normals of the device generator for GLASS indices g of
-    one shell: counter (g lo, g hi, shell lo, shell hi), key (seed lo, seed hi),
-    Box-Muller on u1 = (w0 + 1/2) 2^-32 and u2 = (w1 >> 8) 2^-24 (FP64 here; the
-    device evaluates log / sqrt / sin / cos in FP32)."""
+    one shell: one Philox call per (g, shell pair), counter (g lo, g hi,
+    u lo, u hi) with u = shell // 2, key (seed lo, seed hi); words (0, 1) for an
+    even shell, (2, 3) for an odd one; Box-Muller on u1 = (w_a + 1/2) 2^-32 and
+    u2 = (w_b >> 8) 2^-24 (FP64 here; the device evaluates log / sqrt / sin /
+    cos in FP32)."""
     g = np.asarray(g, dtype=np.uint64)
+    u = shell >> 1
     ctr = np.stack([g & np.uint64(0xFFFFFFFF), g >> np.uint64(32),
-                    np.full_like(g, shell & 0xFFFFFFFF), np.full_like(g, shell >> 32)], axis=-1)
+                    np.full_like(g, u & 0xFFFFFFFF), np.full_like(g, u >> 32)], axis=-1)
     key = np.broadcast_to(np.array([seed & 0xFFFFFFFF, seed >> 32], dtype=np.uint64), ctr.shape[:-1] + (2,))
     w = philox4x32_10(ctr.astype(np.uint32), key.astype(np.uint32))
-    u1 = (np.float32(w[..., 0]).astype(np.float64) + 0.5) * 2.0**-32
-    u2 = (w[..., 1] >> 8).astype(np.float64) * 2.0**-24
+    a = 2 * (shell & 1)
+    u1 = (np.float32(w[..., a]).astype(np.float64) + 0.5) * 2.0**-32
+    u2 = (w[..., a + 1] >> 8).astype(np.float64) * 2.0**-24
     rad = np.sqrt(-2.0 * np.log(u1))
     return rad * np.cos(2 * np.pi * u2) + 1j * rad * np.sin(2 * np.pi * u2)
 

@@ -248,3 +248,15 @@ def test_device_normals_match_oracle(shell, seed):
               _lib.ptr(out), _lib.stream_ptr())
     got = out.cpu().numpy()
     assert np.max(np.abs(got - ref)) < 2e-5
+    # the batched FP32 form (one thread walks the shells; a Philox call serves
+    # an even shell and the odd one after it): three shells from `shell`
+    ns = 3
+    w3 = torch.ones((ns, nl, 1), dtype=torch.float64, device="cuda")
+    o32 = torch.empty((nl * (nl + 1) // 2, ns), dtype=torch.complex64, device="cuda")
+    _lib.call("ga_fields_alm", nl, 0, shell, ns, _lib.ptr(w3), None, 1, seed, _lib.ptr(o32), 0,
+              None, _lib.stream_ptr())
+    got32 = o32.cpu().numpy()
+    for j in range(ns):
+        zj = O.glass_to_healpix_alm(O.device_normals(seed, shell + j, np.arange(nl * (nl + 1) // 2)))
+        zj[:nl] = zj[:nl].real + zj[:nl].imag
+        assert np.max(np.abs(got32[:, j] - zj)) < 2e-5, (shell, j)

@@ -419,10 +419,20 @@ def test_philox_known_answers():
     out = O.philox4x32_10(ctr, key)
     assert out.tolist() == [list(e) for _, _, e in kat]
     # Box-Muller on top: standard complex normals
-    z = O.device_normals(11, 2, np.arange(200000))
-    assert abs(z.real.mean()) < 0.02 and abs(z.imag.mean()) < 0.02
-    assert abs(z.real.var() - 1) < 0.02 and abs(z.imag.var() - 1) < 0.02
-    assert abs(np.mean(z.real * z.imag)) < 0.02
+    for shell in (2, 3):  # words 0, 1 and words 2, 3 of the same calls
+        z = O.device_normals(11, shell, np.arange(200000))
+        assert abs(z.real.mean()) < 0.02 and abs(z.imag.mean()) < 0.02
+        assert abs(z.real.var() - 1) < 0.02 and abs(z.imag.var() - 1) < 0.02
+        assert abs(np.mean(z.real * z.imag)) < 0.02
+    # the two shells of one call are uncorrelated, and shell 3's value at
+    # g = 5 comes from words 2 and 3 of the call with counter (5, 0, 1, 0)
+    z2, z3 = (O.device_normals(11, s_, np.arange(200000)) for s_ in (2, 3))
+    assert abs(np.mean(z2.real * z3.real)) < 0.02 and abs(np.mean(z2.imag * z3.imag)) < 0.02
+    w = O.philox4x32_10(np.array([[5, 0, 1, 0]], dtype=np.uint32), np.array([[11, 0]], dtype=np.uint32))
+    u1 = (np.float64(np.float32(w[0, 2])) + 0.5) * 2.0**-32
+    u2 = float(w[0, 3] >> 8) * 2.0**-24
+    r = np.sqrt(-2.0 * np.log(u1))
+    assert abs(O.device_normals(11, 3, np.array([5]))[0] - r * np.exp(2j * np.pi * u2)) < 1e-12
 
 
 class _UnitDrawRng:
