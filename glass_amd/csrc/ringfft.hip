@@ -870,6 +870,41 @@ __device__ __forceinline__ void load_pair(const cplx<float> *__restrict__ f, cpl
   }
 }
 
+// The rows m = 0..lmax of one unit (two shells) of the ring buffer through a
+// buffer resource whose range ends after the pair of row lmax: a read at
+// `oob`, past that end, returns zeros (buffer range checking), so a row that
+// does not exist needs neither a branch nor a zero fill.  MODE as load_pair.
+template <int MODE>
+struct f_rows {
+  __amdgpu_buffer_rsrc_t rs;
+  uint32_t row, oob;
+  __device__ __forceinline__ f_rows(const cplx<float> *Fu, int G, int lmax) {
+    row = uint32_t(G) * 8u;
+    oob = (uint32_t(lmax) * row + 31u) & ~15u;  // >= the end of row lmax's pair
+    rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<cplx<float> *>(Fu), 0, int(oob),
+                                           0x00020000);
+  }
+  __device__ __forceinline__ void load(int m, bool ok, cplx<float> &a, cplx<float> &b) const {
+    const uint32_t off = ok ? uint32_t(m) * row : oob;
+    if (MODE == 2) {
+      const float4 v =
+          __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+      a = cplx<float>{v.x, v.y};
+      b = cplx<float>{v.z, v.w};
+    } else {
+      const float2 v = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+      a = cplx<float>{v.x, v.y};
+      if (MODE == 1) {
+        const float2 w =
+            __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, off + 8u, 0, 0));
+        b = cplx<float>{w.x, w.y};
+      } else {
+        b = cplx<float>{0.0f, 0.0f};
+      }
+    }
+  }
+};
+
 template <int NT, int MODE>
 __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, int G, int n, int r,
                                           int lmax, int lbits, int ldx, cplx<float> *X) {
@@ -885,6 +920,9 @@ __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, in
   auto pack = [](cplx<float> g0, cplx<float> g1) {
     return cplx<float>{g0.re - g1.im, g0.im + g1.re};
   };
+  // terms that do not exist (m > lmax) read zeros: every term is one load and
+  // one unconditional addition
+  const f_rows<MODE> fr(Fu, G, lmax);
   if (jmax == 0) {
     // n > lmax: a bin has at most one direct and one conjugate term
     for (int k0 = t; k0 < n; k0 += kBatch * NT) {
@@ -892,24 +930,14 @@ __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, in
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int k = min(k0 + u * NT, n - 1);
-        load_pair<MODE>(Fu + int64_t(min(k, lmax)) * G, fd[u][0], fd[u][1]);
-        load_pair<MODE>(Fu + int64_t(min(n - k, lmax)) * G, fc[u][0], fc[u][1]);
+        fr.load(k, k <= lmax, fd[u][0], fd[u][1]);
+        fr.load(n - k, n - k <= lmax, fc[u][0], fc[u][1]);
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int k = k0 + u * NT;
-        if (k < n) {
-          cplx<float> g0 = {0.0f, 0.0f}, g1 = {0.0f, 0.0f};
-          if (k <= lmax) {
-            g0 = fd[u][0];
-            g1 = fd[u][1];
-          }
-          if (n - k <= lmax) {
-            g0 = cadd(g0, cconj(fc[u][0]));
-            g1 = cadd(g1, cconj(fc[u][1]));
-          }
-          X[slot(k)] = pack(g0, g1);
-        }
+        if (k < n)
+          X[slot(k)] = pack(cadd(fd[u][0], cconj(fc[u][0])), cadd(fd[u][1], cconj(fc[u][1])));
       }
     }
     return;
@@ -926,20 +954,15 @@ __device__ __forceinline__ void fold_bins(const cplx<float> *__restrict__ Fu, in
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         const int j = j0 + u * Q;
-        load_pair<MODE>(Fu + int64_t(min(j * n + k, lmax)) * G, fd[u][0], fd[u][1]);
-        load_pair<MODE>(Fu + int64_t(min((j + 1) * n - k, lmax)) * G, fc[u][0], fc[u][1]);
+        fr.load(j * n + k, j * n + k <= lmax, fd[u][0], fd[u][1]);
+        fr.load((j + 1) * n - k, (j + 1) * n - k <= lmax, fc[u][0], fc[u][1]);
       }
 #pragma unroll
       for (int u = 0; u < B; ++u) {
-        const int j = j0 + u * Q;
-        if (j * n + k <= lmax) {
-          d0 = cadd(d0, fd[u][0]);
-          d1 = cadd(d1, fd[u][1]);
-        }
-        if ((j + 1) * n - k <= lmax) {
-          c0 = cadd(c0, cconj(fc[u][0]));
-          c1 = cadd(c1, cconj(fc[u][1]));
-        }
+        d0 = cadd(d0, fd[u][0]);
+        d1 = cadd(d1, fd[u][1]);
+        c0 = cadd(c0, cconj(fc[u][0]));
+        c1 = cadd(c1, cconj(fc[u][1]));
       }
     };
     int j0 = q;
@@ -1228,9 +1251,11 @@ __device__ __forceinline__ void fold_wide_pairs(const cplx<float> *__restrict__ 
   const int t = threadIdx.x;
   constexpr int P = C / 2;
   const bool far = lmax >= 2 * r;  // terms at j >= 2 (block-uniform)
+  const f_rows<MODE> fr(Fu, G, lmax);
 #pragma unroll
   for (int i0 = 0; i0 < P; i0 += B) {
     cplx<float> f[B][2][4][2];  // [q][side a / b][j][shell of the pair]
+    // every term is one load; terms that do not exist read zeros
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       const int q = t + (i0 + u) * NT;
@@ -1240,27 +1265,9 @@ __device__ __forceinline__ void fold_wide_pairs(const cplx<float> *__restrict__ 
       for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          f[u][sd][j][0] = f[u][sd][j][1] = cplx<float>{0.0f, 0.0f};
           const int mm = ab[sd] + r * j;
-          if (j < 2 && okq && mm <= lmax)
-            load_pair<MODE>(Fu + int64_t(mm) * G, f[u][sd][j][0], f[u][sd][j][1]);
+          fr.load(mm, (j < 2 || far) && okq && mm <= lmax, f[u][sd][j][0], f[u][sd][j][1]);
         }
-    }
-    if (far) {
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const int q = t + (i0 + u) * NT;
-        const bool okq = q < (r + 1) / 2;
-        const int ab[2] = {q, q ? r - q : r};
-#pragma unroll
-        for (int sd = 0; sd < 2; ++sd)
-#pragma unroll
-          for (int j = 2; j < 4; ++j) {
-            const int mm = ab[sd] + r * j;
-            if (okq && mm <= lmax)
-              load_pair<MODE>(Fu + int64_t(mm) * G, f[u][sd][j][0], f[u][sd][j][1]);
-          }
-      }
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
@@ -1275,9 +1282,8 @@ __device__ __forceinline__ void fold_wide_pairs(const cplx<float> *__restrict__ 
           cplx<float> c[4][2];
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            c[j][0] = c[j][1] = cplx<float>{0.0f, 0.0f};
             const int mm = r / 2 + r * j;
-            if (valid && mm <= lmax) load_pair<MODE>(Fu + int64_t(mm) * G, c[j][0], c[j][1]);
+            fr.load(mm, valid && mm <= lmax, c[j][0], c[j][1]);
           }
 #pragma unroll
           for (int k2 = 0; k2 < 4; ++k2)
